@@ -1,0 +1,150 @@
+/*
+ * slgpu — MI355X (gfx950) implementation of SuperLite's per-iteration hot path,
+ * behind a plain C ABI callable from Fortran through iso_c_binding.
+ *
+ * Each entry point replaces the body of one reference subroutine (or a fixed
+ * sequence of them, in the order superlite.f90:157-219 calls them).  The host keeps
+ * every module array it owns; the library keeps device-resident copies of the mesh,
+ * groups, line list, opacities, DDMC tables, particles and tallies between calls.
+ *
+ * Conventions
+ *   - every function returns 0 on success, a positive SLGPU_E* code on error;
+ *     slgpu_last_error_message() gives a text description.
+ *   - cells ic = 1..nx map to 0-based storage; the mesh has no void cells (ic == ix).
+ *   - group-resolved arrays use the reference's memory order grd_cap(ng, nc):
+ *     element (ig, ic) is at [(ic-1)*ng + (ig-1)].
+ *   - output pointers are host pointers and may be NULL (the result then stays on
+ *     the device for the next call); input pointers are host pointers, borrowed for
+ *     the duration of the call only.
+ *   - one context per process and GPU; calls on one context are host-serialised and
+ *     stream-ordered on the context's HIP stream; a context is not thread-safe.
+ */
+#ifndef SLGPU_H
+#define SLGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slgpu_ctx slgpu_ctx;
+
+enum {
+    SLGPU_OK = 0,
+    SLGPU_EINVAL = 1,      /* bad argument / size */
+    SLGPU_EORDER = 2,      /* call made before its inputs were set */
+    SLGPU_EHIP = 3,        /* HIP runtime error */
+    SLGPU_ETRANSPORT = 4,  /* transport error with errorfatal set (particle_advance.f90:262-266) */
+    SLGPU_ENOMEM = 5
+};
+
+enum { SLGPU_FP64 = 0, SLGPU_MIXED = 1 };
+
+/* opacity flags (inputparmod.f:63-66: in_nobbopac, in_nobfopac, in_noffopac, in_nothmson) */
+enum { SLGPU_NOBB = 1, SLGPU_NOBF = 2, SLGPU_NOFF = 4, SLGPU_NOTHOMSON = 8 };
+
+typedef struct {
+    int nx;                /* grd_nx == grd_ncell (gridmod.f:15,33) */
+    int ng;                /* grp_ng (groupmod.f:7) */
+    int nep, nepg;         /* grd_nep, grd_nepg (gridmod.f:121-122) */
+    double tauddmc;        /* trn_tauddmc (transportmod.f:14) */
+    double taulump;        /* trn_taulump (transportmod.f:15) */
+    int puretran;          /* in_puretran (inputparmod.f:56) */
+    int errorfatal;        /* trn_errorfatal (transportmod.f:11) */
+    int precision;         /* SLGPU_FP64 or SLGPU_MIXED */
+    uint32_t seed;         /* key of the counter-based per-particle RNG (replaces randommod.f) */
+    int64_t npartmax;      /* prt_npartmax (particlemod.f90:175) */
+    int64_t max_steps;     /* per-particle step cap, a particle beyond it dies with error 200 (0: 2^40) */
+} slgpu_config;
+
+/* Host gas state consumed by the opacity build (gasmod.f:20-34 and ion_grndlev,
+ * ionsmod.f:35-40, packed ion-fastest for the ion list given to slgpu_set_ions). */
+typedef struct {
+    const double *temp;     /* gas_temp(nc) */
+    const double *natom;    /* gas_natom(nc) */
+    const double *vol;      /* gas_vol(nc) */
+    const double *mass;     /* gas_mass(nc) */
+    const double *nelec;    /* gas_nelec(nc) */
+    const double *natomfr;  /* gas_natomfr(1:30, nc), iz-fastest */
+    const double *oc;       /* ion_grndlev(iz,ic)%oc(ii)   as [(ic-1)*nion + k] */
+    const double *ginv;     /* ion_grndlev(iz,ic)%ginv(ii) as [(ic-1)*nion + k] */
+} slgpu_gas;
+
+/* transportmod_init + gridmod/particle allocation (superlite.f90:100-116) */
+int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out);
+int slgpu_destroy(slgpu_ctx *ctx);
+const char *slgpu_last_error_message(void);
+
+/* mesh: grd_xarr(nx+1), grd_vxarr(nx+1), grd_divv(nx), grd_vol(nx), grd_rcell(nx)
+ * (GRID/grid_setup.f:16-51, GRID/grid_volume.f:80-101) */
+int slgpu_set_grid(slgpu_ctx *ctx, const double *xarr, const double *vxarr,
+                   const double *divv, const double *vol, const double *rcell);
+
+/* groups: grp_wl(ng+1) (groupmod.f:45-51); the flux grid is the group grid
+ * (in_flx_ndim(1) == 0, fluxmod.f:132-136) */
+int slgpu_set_groups(slgpu_ctx *ctx, const double *grp_wl);
+
+/* lambda-sorted line list bb_xs (bbxsmod.f:27-34, sorted by bbxsmod.f:126-179) */
+int slgpu_set_lines(slgpu_ctx *ctx, int64_t nline, const float *wl0, const float *gxs,
+                    const float *chilw, const int16_t *iz, const int16_t *ii);
+
+/* Verner tables bf_ph1(6,7,30,30), bf_ph2(7,30,30) (bfxsmod.f:14) and
+ * Sutherland ff_gff(81,41) (ffxsmod.f:18), Fortran order */
+int slgpu_set_bf_tables(slgpu_ctx *ctx, const float *ph1, const float *ph2);
+int slgpu_set_ff_table(slgpu_ctx *ctx, const double *gff);
+
+/* ion list of the populations: (iz, ii) pairs, ascending, ii <= min(iz, ni(iz)-1)
+ * (physical_opacity.f:95-101,165-171) */
+int slgpu_set_ions(slgpu_ctx *ctx, int nion, const int *ion_iz, const int *ion_ii);
+
+/* GAS/physical_opacity.f:3-357 (grp_ngs == 1) + GAS/opacity_planckmean.f:18-24 +
+ * GAS/opacity_rossmean.f:18-21.  out_cap64 is the FP64 `cap` before the sngl() of
+ * physical_opacity.f:267; out_cap32 is gas_cap; out_emiss is gas_emiss. */
+int slgpu_opacity_build(slgpu_ctx *ctx, const slgpu_gas *gas, int flags,
+                        double *out_cap64, float *out_cap32, double *out_sig,
+                        double *out_capgrey, double *out_capross, float *out_emiss);
+
+/* mpimod_ser.f:98-112 (grd_* = gas_*) + SOURCE/sourceenergy.f90:23-50 +
+ * GRID/leakage_opacity11.f90:30-171 + GRID/emission_probability.f90:39-62, plus the
+ * per-cell sampling tables the transport uses in place of the reference's
+ * per-event O(ng) scans.  radtemp = grd_radtemp(nc), lbol = in_L_bol. */
+int slgpu_ddmc_prep(slgpu_ctx *ctx, const double *radtemp, double lbol,
+                    double *out_opaclump /* 11*nc */, double *out_emitprob /* nep*nc */,
+                    double *out_emit /* nc */, double *out_sthermal);
+
+/* SOURCE/sourcenumbers.f90:26-169 (one rank, in_srcepwr = 1, no surface source),
+ * SOURCE/vacancies.f90, SOURCE/interior_source.f90:57-168,
+ * SOURCE/source_transformdirection.f90:23-46.  nstot = src_ns. */
+int slgpu_sources(slgpu_ctx *ctx, int64_t nstot, uint32_t iteration,
+                  int *out_nvol /* nc, may be NULL */, int64_t *out_nnew);
+
+/* particle_advance.f90:65-325 with TRANSPORT1/transport11.f90 and diffusion11.f90.
+ * out_totals = [tot_eout, tot_evelo, tot_sflux, e_start, e_dead, e_source]
+ * out_counters = [npactive, npstepimc, npstepddmc, npstepmax, npmethswap, npflux,
+ *                 ndead, nerror, nwarning, 0] (countersmod.f:12-20) */
+int slgpu_transport(slgpu_ctx *ctx, uint32_t iteration, double *out_tally /* nc */,
+                    double *out_jrad /* ng*nc */, double *out_totals /* 6 */,
+                    int64_t *out_counters /* 10 */);
+
+/* fluxtally.f:25-54: flx_luminos, flx_lumdev, flx_lumnum over the group grid */
+int slgpu_fluxtally(slgpu_ctx *ctx, double *out_luminos, double *out_lumdev,
+                    int *out_lumnum);
+
+/* per-particle final state of the last transport (stat 1 flux, 2 dead), host copies;
+ * any pointer may be NULL.  n = number of particles of the last slgpu_sources. */
+int slgpu_get_particles(slgpu_ctx *ctx, int64_t n, uint8_t *stat, double *x, double *mu,
+                        double *wl, double *e, int *ig, int64_t *nstep, int *err);
+
+/* accumulated device time per phase in ms, measured with HIP events on the context
+ * stream: [0] opacity, [1] ddmc prep, [2] sources, [3] transport, [4] fluxtally,
+ * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone; reset = 1 zeroes them */
+int slgpu_timers(slgpu_ctx *ctx, double *ms, int n, int reset);
+
+/* the HIP stream all work of the context is ordered on (hipStream_t) */
+void *slgpu_stream(slgpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,116 @@
+/* ORACLE — CPU FP64 restatement of SuperLite's per-iteration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Imported, linked or executed only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg, always as the checker or
+ * the reported CPU baseline, never as the product path.
+ *
+ * Each function cites the reference file:line it restates.  Indices in the model
+ * struct follow the reference: cells ix = 1..nx (compressed cell ic == ix, no void
+ * cells), groups ig = 1..ng; arrays are stored 0-based with the reference's memory
+ * order (grd_cap(ng, nc) group-fastest: cap[(ic-1)*ng + ig-1]).
+ *
+ * DDMC per-cell caches (TRANSPORT1/diffusion11.f90:95-152, 275-280, 565-569) are
+ * evaluated exactly for the particle's current cell on every event ("perfect cache"):
+ * the Planck group integrals specarr(ig) always hold specintv(tempinv(ic)), which is
+ * what the reference's cache holds once filled.  This removes the reference's
+ * dependence on thread scheduling and stale entries (the reference reads a
+ * not-yet-filled cache in the unlumped Doppler rate, diffusion11.f90:185-190).
+ */
+#ifndef SL_ORACLE_H
+#define SL_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* grid (gridmod.f:19-23, grid_setup.f:46-51, grid_volume.f:82-87) */
+    int nx;
+    const double *xarr, *vxarr;   /* nx+1 */
+    const double *divv, *vol;     /* nx */
+    /* groups (groupmod.f:45-51) */
+    int ng;
+    const double *wl, *wlinv;     /* ng+1 */
+    int nep, nepg;                /* gridmod.f:121-122 */
+    /* transport parameters (inputparmod.f:52-57) */
+    double tauddmc, taulump;
+    int puretran;
+    int max_steps;                /* per-particle step cap (0 = none) */
+    /* per-iteration state (grd_* after bcast_nonpermanent, mpimod_ser.f:98-112) */
+    const float *cap;             /* ng*nc */
+    const double *sig, *capgrey, *tempinv;   /* nc */
+    const double *opaclump;       /* 11*nc */
+    const double *emitprob;       /* nep*nc */
+    const double *emit;           /* nc, after sourceenergy */
+    const double *radtemp;        /* nc */
+    const int *nvol;              /* nc, after sourcenumbers */
+} or_model;
+
+typedef struct {
+    double *tally;    /* nc  grd_tally */
+    double *jrad;     /* ng*nc grd_jrad */
+    double *luminos, *lumdev;   /* ng (flux grid == group grid) */
+    int *lumnum;
+    double tot[6];    /* eout, evelo, sflux, estart(lab, after the IMC lab transform), edead, esrc(comoving) */
+    int64_t ct[10];   /* nactive, nstepimc, nstepddmc, nstepmax, nmethswap, nflux, ndead, nerr, nwarn, 0 */
+    /* optional per-particle final state (NULL to skip) */
+    uint8_t *pstat;   /* 1 flux, 2 dead */
+    double *px, *pmu, *pwl, *pe;
+    int *pig, *pix;
+    int64_t *pnstep;
+    int *perr;
+} or_result;
+
+/* ---- building blocks --------------------------------------------------- */
+int or_binsrch(double x, const double *arr, int n, int widerange);          /* MISC/binsrch.f90:3-77 */
+void or_specintv(double tempinv, const double *wlinv, int n, int offset, double *ss);  /* groupmod.f:146-272 mode 0 */
+void or_spectintv(double tempinv, const double *wlinv, int n, double *ss);  /* groupmod.f:275-404 mode 0 */
+double or_specint0(double tempinv, const double *wlinv, int ig);            /* groupmod.f:99-120 */
+double or_dopspeccalc(double tempinv, const double *wlinv, int ig);         /* groupmod.f:123-143 */
+double or_bfxs(int nz, int ne, double e, const float *ph1, const float *ph2); /* bfxsmod.f:65-153 */
+double or_planck(double wl, double temp);                                   /* physconstmod.f:79-93 */
+
+/* ---- opacity build: physical_opacity.f:3-357 + opacity_planckmean.f + opacity_rossmean.f ---- */
+typedef struct {
+    int nc, ng;
+    const double *wl;                 /* ng+1 grp_wl, cm */
+    const double *temp, *natom, *vol, *mass, *nelec;   /* nc */
+    const double *natomfr;            /* 30*nc, (iz,ic) iz-fastest */
+    int nion;
+    const int *ion_iz, *ion_ii;       /* nion, ascending (iz, ii) */
+    const double *oc, *ginv;          /* nion*nc ion-fastest */
+    int nline;
+    const float *wl0, *gxs, *chilw;   /* nline, lambda-sorted */
+    const int16_t *liz, *lii;
+    const float *bf_ph1, *bf_ph2;     /* (6,7,30,30), (7,30,30) Fortran order */
+    const double *ff_gff;             /* (81,41) Fortran order */
+    int flags;                        /* bit0 no bb, bit1 no bf, bit2 no ff, bit3 no thomson */
+} or_opac_in;
+
+int or_opacity(const or_opac_in *in, double *cap64 /* ng*nc group-fastest */, float *cap32,
+               double *sig, double *capgrey, double *capross, float *emiss);
+
+/* ---- DDMC prep: leakage_opacity11.f90:30-171, emission_probability.f90:39-62 ---- */
+void or_leakage(const or_model *m, double *opaclump);
+void or_emission_probability(const or_model *m, double *emitprob);
+int or_emitgroup(const or_model *m, double r, int ic);                       /* GRID/emitgroup.f:3-54 */
+
+/* ---- sources: sourceenergy.f90:23-50, sourcenumbers.f90:26-169 (nmpi=1, no surface source) ---- */
+void or_sourceenergy(int nc, const double *rcell, const double *radtemp, double lbol,
+                     double *emit, double *sthermal);
+int64_t or_sourcenumbers(int nc, const double *emit, int64_t nstot, int *nvol);
+
+/* ---- one Monte Carlo iteration: interior_source + source_transformdirection +
+ *      particle_advance (transport11/diffusion11) + fluxtally, Philox streams
+ *      key (seed, iteration), counter (particle, block, phase 0=source 1=transport) ---- */
+int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result *res);
+
+/* source sampling only (interior_source.f90:57-168 + source_transformdirection.f90:23-46) */
+int or_sources(const or_model *m, uint32_t seed, uint32_t iteration,
+               double *x, double *mu, double *om, double *e, double *e0, double *wl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

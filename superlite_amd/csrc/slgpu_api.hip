@@ -1,0 +1,625 @@
+// C ABI of libslgpu.so (include/slgpu.h): context, uploads, per-iteration entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "slgpu_device.h"
+#include "slgpu_internal.h"
+
+using namespace slgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SLGPU_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(slgpu_ctx *c, T **p, size_t n)
+{
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(SLGPU_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    (void)hipMemsetAsync(q, 0, n * sizeof(T), c->stream);
+    c->allocations.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+void dfree(slgpu_ctx *c, void *p)
+{
+    if (!p) return;
+    auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
+    if (it != c->allocations.end()) c->allocations.erase(it);
+    hipFree(p);
+}
+
+struct PhaseTimer {
+    slgpu_ctx *c;
+    int slot;
+    PhaseTimer(slgpu_ctx *c_, int s) : c(c_), slot(s) { hipEventRecord(c->ev[2 * s], c->stream); }
+    int stop()
+    {
+        hipEventRecord(c->ev[2 * slot + 1], c->stream);
+        hipError_t e = hipEventSynchronize(c->ev[2 * slot + 1]);
+        if (e != hipSuccess) return fail(SLGPU_EHIP, std::string("device: ") + hipGetErrorString(e));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev[2 * slot], c->ev[2 * slot + 1]);
+        c->t_ms[slot] += ms;
+        return 0;
+    }
+};
+
+// SOURCE/sourcenumbers.f90:83-145 with one rank, in_srcepwr = 1, tot_esurf = 0
+int64_t source_numbers(const std::vector<double> &emit, int64_t nstot, std::vector<int> &nvol)
+{
+    const int nc = (int)emit.size();
+    double etot = 0.0;
+    int64_t ncactive = 0;
+    for (double e : emit) {
+        etot += std::pow(e, 1.0);
+        if (e > 0.0) ncactive++;
+    }
+    nvol.assign(nc, 0);
+    if (ncactive == 0) return -1;
+    const int64_t nsavail = nstot;
+    const double invn = 1.0 / (double)nsavail;
+    const double einv = etot == 0.0 ? 0.0 : 1.0 / etot;
+    double edone = 0.0;
+    int64_t ndone = 0;
+    for (int l = 0; l < nc; ++l) {
+        const double en = std::pow(emit[l], 1.0);
+        if (en == 0.0) continue;
+        int64_t n = std::max<int64_t>(1, (int64_t)(en * (double)nsavail * einv));
+        if (edone * einv > (double)ndone * invn) n = n + 1;
+        nvol[l] = (int)n;
+        edone += en;
+        ndone += n;
+    }
+    int64_t s = 0;
+    for (int v : nvol) s += v;
+    int64_t nextra = nstot - s;
+    if (nextra > nc) return -2;       // sourcenumbers.f90:130-131
+    const int64_t nsmean = nstot / ncactive;
+    while (nextra != 0) {
+        for (int l = 0; l < nc; ++l) {
+            if (nextra == 0) break;
+            if (nvol[l] < nsmean) continue;
+            const int sg = nextra > 0 ? 1 : -1;
+            nvol[l] += sg;
+            nextra -= sg;
+        }
+    }
+    s = 0;
+    for (int v : nvol) s += v;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *slgpu_last_error_message(void) { return g_err.c_str(); }
+
+int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
+{
+    if (!cfg || !out) return fail(SLGPU_EINVAL, "null argument");
+    if (cfg->nx < 1 || cfg->nx > 8192) return fail(SLGPU_EINVAL, "nx must be in 1..8192");
+    if (cfg->ng < 2 || cfg->ng > 32767) return fail(SLGPU_EINVAL, "ng must be in 2..32767 (groupmod.f:38)");
+    if (cfg->nep < 1 || cfg->nep > 1024 || cfg->nepg < 1)
+        return fail(SLGPU_EINVAL, "nep must be in 1..1024 and nepg >= 1");
+    if (cfg->npartmax < 1 || cfg->npartmax > (int64_t)1 << 31)
+        return fail(SLGPU_EINVAL, "npartmax must be in 1..2^31");
+    if (cfg->precision != SLGPU_FP64 && cfg->precision != SLGPU_MIXED)
+        return fail(SLGPU_EINVAL, "precision must be SLGPU_FP64 or SLGPU_MIXED");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(SLGPU_EHIP, "no HIP device visible");
+    slgpu_ctx *c = new slgpu_ctx();
+    c->cfg = *cfg;
+    c->nx = cfg->nx;
+    c->ng = cfg->ng;
+    Dev &d = c->d;
+    std::memset(&d, 0, sizeof(d));
+    d.nx = cfg->nx;
+    d.ng = cfg->ng;
+    d.nep = cfg->nep;
+    d.nepg = cfg->nepg;
+    d.tauddmc = cfg->tauddmc;
+    d.taulump = cfg->taulump;
+    d.puretran = cfg->puretran;
+    d.precision = cfg->precision;
+    d.max_steps = cfg->max_steps > 0 ? cfg->max_steps : ((int64_t)1 << 40);
+    d.seed = cfg->seed;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SLGPU_EHIP, "hipStreamCreate failed");
+    }
+    for (int k = 0; k < 16; ++k) hipEventCreate(&c->ev[k]);
+    const size_t nc = c->nx, ng = c->ng, cg = nc * ng, np = cfg->npartmax;
+    int rc = 0;
+    rc |= dalloc(c, (double **)&d.xarr, nc + 1);
+    rc |= dalloc(c, (double **)&d.vxarr, nc + 1);
+    rc |= dalloc(c, (double **)&d.divv, nc);
+    rc |= dalloc(c, (double **)&d.vol, nc);
+    rc |= dalloc(c, (double **)&d.wl, ng + 1);
+    rc |= dalloc(c, (double **)&d.wlinv, ng + 1);
+    rc |= dalloc(c, (double **)&d.gff, 81 * 41);
+    rc |= dalloc(c, &d.hckt, nc);
+    rc |= dalloc(c, &d.hlparr, nc);
+    rc |= dalloc(c, &d.sig, nc);
+    rc |= dalloc(c, &d.tempinv, nc);
+    rc |= dalloc(c, &d.temp, nc);
+    rc |= dalloc(c, &d.mass, nc);
+    rc |= dalloc(c, &d.natomfr, 30 * nc);
+    rc |= dalloc(c, &d.el_z, 30);
+    rc |= dalloc(c, &d.capT, cg);
+    rc |= dalloc(c, &d.cap64, cg);
+    rc |= dalloc(c, &d.cap32, cg);
+    rc |= dalloc(c, &d.emiss, cg);
+    rc |= dalloc(c, &d.capgrey, nc);
+    rc |= dalloc(c, &d.capross, nc);
+    rc |= dalloc(c, &d.specarr, cg);
+    rc |= dalloc(c, &d.opaclump, 11 * nc);
+    rc |= dalloc(c, &d.emitprob, (size_t)cfg->nep * nc);
+    rc |= dalloc(c, &d.emitcum, cg);
+    rc |= dalloc(c, &d.emit, nc);
+    rc |= dalloc(c, &d.radtemp, nc);
+    rc |= dalloc(c, &d.srccum, cg);
+    rc |= dalloc(c, &d.cell, nc);
+    rc |= dalloc(c, &d.gorder, cg);
+    rc |= dalloc(c, &d.cumL, cg);
+    rc |= dalloc(c, &d.cumR, cg);
+    rc |= dalloc(c, &d.cumD, cg);
+    rc |= dalloc(c, &d.cumS, cg);
+    rc |= dalloc(c, (int64_t **)&d.pstart, nc + 1);
+    rc |= dalloc(c, &d.px, np);
+    rc |= dalloc(c, &d.pmu, np);
+    rc |= dalloc(c, &d.pwl, np);
+    rc |= dalloc(c, &d.pe, np);
+    rc |= dalloc(c, &d.pe0, np);
+    rc |= dalloc(c, &d.pstat, np);
+    rc |= dalloc(c, &d.pig, np);
+    rc |= dalloc(c, &d.perr, np);
+    rc |= dalloc(c, &d.pnstep, np);
+    rc |= dalloc(c, &d.tally, nc);
+    rc |= dalloc(c, &d.jrad, cg);
+    rc |= dalloc(c, &d.tot, 8);
+    rc |= dalloc(c, &d.ct, 10);
+    rc |= dalloc(c, &d.queue, 1);
+    rc |= dalloc(c, &d.lum, ng);
+    rc |= dalloc(c, &d.dev, ng);
+    rc |= dalloc(c, &d.num, ng);
+    rc |= dalloc(c, &d.flags, 1);
+    if (rc) {
+        std::string m = g_err;
+        slgpu_destroy(c);
+        return fail(SLGPU_ENOMEM, m);
+    }
+    hipStreamSynchronize(c->stream);
+    *out = c;
+    return 0;
+}
+
+int slgpu_destroy(slgpu_ctx *c)
+{
+    if (!c) return 0;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->allocations) hipFree(p);
+    c->allocations.clear();
+    for (int k = 0; k < 16; ++k) hipEventDestroy(c->ev[k]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+void *slgpu_stream(slgpu_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int slgpu_set_grid(slgpu_ctx *c, const double *xarr, const double *vxarr, const double *divv,
+                   const double *vol, const double *rcell)
+{
+    if (!c || !xarr || !vxarr || !divv || !vol || !rcell) return fail(SLGPU_EINVAL, "null argument");
+    const int nx = c->nx;
+    for (int i = 0; i < nx; ++i)
+        if (!(xarr[i + 1] > xarr[i])) return fail(SLGPU_EINVAL, "grd_xarr must increase");
+    for (int i = 0; i < nx; ++i)
+        if (!(vol[i] > 0.0)) return fail(SLGPU_EINVAL, "grd_vol must be positive");
+    c->h_xarr.assign(xarr, xarr + nx + 1);
+    c->h_vxarr.assign(vxarr, vxarr + nx + 1);
+    c->h_divv.assign(divv, divv + nx);
+    c->h_vol.assign(vol, vol + nx);
+    c->h_rcell.assign(rcell, rcell + nx);
+    HIPCHK(hipMemcpyAsync((void *)c->d.xarr, xarr, (nx + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void *)c->d.vxarr, vxarr, (nx + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void *)c->d.divv, divv, nx * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void *)c->d.vol, vol, nx * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_grid = true;
+    return 0;
+}
+
+static int upload_lines(slgpu_ctx *c);
+static int build_bfxs(slgpu_ctx *c);
+
+int slgpu_set_groups(slgpu_ctx *c, const double *wl)
+{
+    if (!c || !wl) return fail(SLGPU_EINVAL, "null argument");
+    const int ng = c->ng;
+    for (int g = 0; g < ng; ++g)
+        if (!(wl[g + 1] > wl[g]) || !(wl[g] > 0.0)) return fail(SLGPU_EINVAL, "grp_wl must be positive and increasing");
+    c->h_wl.assign(wl, wl + ng + 1);
+    std::vector<double> inv(ng + 1);
+    for (int g = 0; g <= ng; ++g) inv[g] = 1.0 / wl[g];     // groupmod.f:50-51
+    HIPCHK(hipMemcpyAsync((void *)c->d.wl, wl, (ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void *)c->d.wlinv, inv.data(), (ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_groups = true;
+    int rc = 0;
+    if (c->have_lines && c->have_ions) rc = upload_lines(c);
+    if (!rc && c->have_bf && c->have_ions) rc = build_bfxs(c);
+    return rc;
+}
+
+int slgpu_set_ions(slgpu_ctx *c, int nion, const int *ion_iz, const int *ion_ii)
+{
+    if (!c || nion < 0 || (nion > 0 && (!ion_iz || !ion_ii))) return fail(SLGPU_EINVAL, "bad ion list");
+    for (int k = 0; k < nion; ++k) {
+        if (ion_iz[k] < 1 || ion_iz[k] > 30 || ion_ii[k] < 1 || ion_ii[k] > ion_iz[k])
+            return fail(SLGPU_EINVAL, "ion (iz, ii) out of range");
+        if (k > 0 && (ion_iz[k] < ion_iz[k - 1] || (ion_iz[k] == ion_iz[k - 1] && ion_ii[k] <= ion_ii[k - 1])))
+            return fail(SLGPU_EINVAL, "ion list must be ascending in (iz, ii)");
+    }
+    c->nion = nion;
+    c->d.nion = nion;
+    c->h_ion_iz.assign(ion_iz, ion_iz + nion);
+    c->h_ion_ii.assign(ion_ii, ion_ii + nion);
+    dfree(c, c->d.ocg);
+    dfree(c, c->d.ocT);
+    dfree(c, (void *)c->d.bfxs);
+    const size_t nc = c->nx;
+    int rc = 0;
+    rc |= dalloc(c, &c->d.ocg, (size_t)std::max(nion, 1) * nc);
+    rc |= dalloc(c, &c->d.ocT, (size_t)(nion + 30) * nc);     // tail: ff coordinates per element
+    rc |= dalloc(c, (double **)&c->d.bfxs, (size_t)std::max(nion, 1) * c->ng);
+    if (rc) return rc;
+    c->have_ions = true;
+    if (c->have_lines && c->have_groups) rc = upload_lines(c);
+    if (!rc && c->have_bf && c->have_groups) rc = build_bfxs(c);
+    return rc;
+}
+
+int slgpu_set_lines(slgpu_ctx *c, int64_t nline, const float *wl0, const float *gxs,
+                    const float *chilw, const int16_t *iz, const int16_t *ii)
+{
+    if (!c || nline < 0 || (nline > 0 && (!wl0 || !gxs || !chilw || !iz || !ii)))
+        return fail(SLGPU_EINVAL, "bad line list");
+    if (nline > ((int64_t)1 << 31) - 1) return fail(SLGPU_EINVAL, "too many lines");
+    for (int64_t l = 1; l < nline; ++l)
+        if (wl0[l] < wl0[l - 1]) return fail(SLGPU_EINVAL, "line list must be sorted by wl0 (bbxsmod.f:126-179)");
+    c->nline = nline;
+    c->h_wl0.assign(wl0, wl0 + nline);
+    c->h_gxs.assign(gxs, gxs + nline);
+    c->h_chilw.assign(chilw, chilw + nline);
+    c->h_liz.assign(iz, iz + nline);
+    c->h_lii.assign(ii, ii + nline);
+    c->have_lines = true;
+    if (c->have_groups && c->have_ions) return upload_lines(c);
+    return 0;
+}
+
+// Lines of group g (1-based) are those with wlsub(g) <= wl0 < wlsub(g+1), the
+// pointer walk of physical_opacity.f:118-125 on a sorted list; lines outside the grid and
+// lines of ions without populations are dropped (they add nothing, physical_opacity.f:146).
+static int upload_lines(slgpu_ctx *c)
+{
+    const int ng = c->ng;
+    int lut[31][32];
+    for (auto &r : lut)
+        for (int &v : r) v = -1;
+    for (int k = 0; k < c->nion; ++k) lut[c->h_ion_iz[k]][c->h_ion_ii[k]] = k;
+    std::vector<LineRec> recs;
+    recs.reserve(c->nline);
+    std::vector<int> cnt(ng + 2, 0);
+    int g = 1;
+    for (int64_t l = 0; l < c->nline; ++l) {
+        const double w = (double)c->h_wl0[l] * PC_ANG;
+        if (w < c->h_wl[0] || !(w < c->h_wl[ng])) continue;
+        while (g < ng && !(c->h_wl[g] > w)) ++g;      // wlsub(g) <= w < wlsub(g+1)
+        const int zi = c->h_liz[l], ii = c->h_lii[l];
+        const int k = (zi >= 1 && zi <= 30 && ii >= 1 && ii <= 31) ? lut[zi][ii] : -1;
+        if (k < 0) continue;
+        recs.push_back(LineRec{c->h_gxs[l], c->h_chilw[l], k, 0});
+        cnt[g]++;
+    }
+    std::vector<int> gstart(ng + 2, 0);
+    for (int q = 1; q <= ng; ++q) gstart[q + 1] = gstart[q] + cnt[q];
+    c->nline_used = (int64_t)recs.size();
+    dfree(c, (void *)c->d.lines);
+    dfree(c, (void *)c->d.gstart);
+    int rc = 0;
+    rc |= dalloc(c, (LineRec **)&c->d.lines, recs.size());
+    rc |= dalloc(c, (int **)&c->d.gstart, ng + 2);
+    if (rc) return rc;
+    if (!recs.empty())
+        HIPCHK(hipMemcpyAsync((void *)c->d.lines, recs.data(), recs.size() * sizeof(LineRec),
+                              hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void *)c->d.gstart, gstart.data(), (ng + 2) * sizeof(int),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int build_bfxs(slgpu_ctx *c)
+{
+    float *d1 = nullptr, *d2 = nullptr;
+    int *diz = nullptr, *dii = nullptr;
+    int rc = 0;
+    rc |= dalloc(c, &d1, c->h_ph1.size());
+    rc |= dalloc(c, &d2, c->h_ph2.size());
+    rc |= dalloc(c, &diz, std::max(c->nion, 1));
+    rc |= dalloc(c, &dii, std::max(c->nion, 1));
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(d1, c->h_ph1.data(), c->h_ph1.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d2, c->h_ph2.data(), c->h_ph2.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (c->nion > 0) {
+        HIPCHK(hipMemcpyAsync(diz, c->h_ion_iz.data(), c->nion * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dii, c->h_ion_ii.data(), c->nion * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    launch_bfxs(c, d1, d2, diz, dii);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, d1); dfree(c, d2); dfree(c, diz); dfree(c, dii);
+    return 0;
+}
+
+int slgpu_set_bf_tables(slgpu_ctx *c, const float *ph1, const float *ph2)
+{
+    if (!c || !ph1 || !ph2) return fail(SLGPU_EINVAL, "null argument");
+    c->h_ph1.assign(ph1, ph1 + 6 * 7 * 30 * 30);
+    c->h_ph2.assign(ph2, ph2 + 7 * 30 * 30);
+    c->have_bf = true;
+    if (c->have_groups && c->have_ions) return build_bfxs(c);
+    return 0;
+}
+
+int slgpu_set_ff_table(slgpu_ctx *c, const double *gff)
+{
+    if (!c || !gff) return fail(SLGPU_EINVAL, "null argument");
+    HIPCHK(hipMemcpyAsync((void *)c->d.gff, gff, 81 * 41 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_ff = true;
+    return 0;
+}
+
+int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *out_cap64,
+                        float *out_cap32, double *out_sig, double *out_capgrey,
+                        double *out_capross, float *out_emiss)
+{
+    if (!c || !gas || !gas->temp || !gas->natom || !gas->vol || !gas->mass || !gas->nelec ||
+        !gas->natomfr || (c->nion > 0 && (!gas->oc || !gas->ginv)))
+        return fail(SLGPU_EINVAL, "null gas state");
+    if (!(c->have_groups && c->have_lines && c->have_bf && c->have_ff && c->have_ions))
+        return fail(SLGPU_EORDER, "groups, lines, ions and bf/ff tables must be set first");
+    const int nc = c->nx, ng = c->ng, nion = c->nion;
+    for (int i = 0; i < nc; ++i)
+        if (!(gas->temp[i] > 0.0)) return fail(SLGPU_EINVAL, "gas_temp must be positive (gas_update.f:71-72)");
+    // present elements (ff adds exactly zero for the others, physical_opacity.f:240-241)
+    int nel = 0, elz[30];
+    for (int z = 1; z <= 30; ++z) {
+        bool any = false;
+        for (int i = 0; i < nc && !any; ++i) any = gas->natomfr[(size_t)i * 30 + z - 1] != 0.0;
+        if (any) elz[nel++] = z;
+    }
+    c->d.nel = nel;
+    PhaseTimer tm(c, 0);
+    // per-call host inputs (small: nc and nion*nc doubles)
+    double *dn = nullptr, *dv = nullptr, *de = nullptr, *doc = nullptr, *dgi = nullptr;
+    int rc = 0;
+    rc |= dalloc(c, &dn, nc);
+    rc |= dalloc(c, &dv, nc);
+    rc |= dalloc(c, &de, nc);
+    rc |= dalloc(c, &doc, (size_t)std::max(nion, 1) * nc);
+    rc |= dalloc(c, &dgi, (size_t)std::max(nion, 1) * nc);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d.temp, gas->temp, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.mass, gas->mass, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.natomfr, gas->natomfr, (size_t)nc * 30 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.el_z, elz, nel * sizeof(int) + 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dn, gas->natom, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dv, gas->vol, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(de, gas->nelec, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (nion > 0) {
+        HIPCHK(hipMemcpyAsync(doc, gas->oc, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dgi, gas->ginv, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    launch_cell_prep(c, dn, dv, de, doc, dgi, flags);
+    hipEventRecord(c->ev[10], c->stream);
+    launch_opacity(c, flags);
+    hipEventRecord(c->ev[11], c->stream);
+    launch_opacity_finish(c);
+    HIPCHK(hipGetLastError());
+    const size_t cg = (size_t)nc * ng;
+    if (out_cap64) HIPCHK(hipMemcpyAsync(out_cap64, c->d.cap64, cg * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_cap32) HIPCHK(hipMemcpyAsync(out_cap32, c->d.cap32, cg * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out_emiss) HIPCHK(hipMemcpyAsync(out_emiss, c->d.emiss, cg * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out_sig) HIPCHK(hipMemcpyAsync(out_sig, c->d.sig, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_capgrey) HIPCHK(hipMemcpyAsync(out_capgrey, c->d.capgrey, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_capross) HIPCHK(hipMemcpyAsync(out_capross, c->d.capross, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    rc = tm.stop();
+    float kms = 0.f;
+    hipEventElapsedTime(&kms, c->ev[10], c->ev[11]);
+    c->t_ms[5] += kms;
+    dfree(c, dn); dfree(c, dv); dfree(c, de); dfree(c, doc); dfree(c, dgi);
+    if (rc) return rc;
+    c->have_opacity = true;
+    c->have_prep = false;
+    return 0;
+}
+
+int slgpu_ddmc_prep(slgpu_ctx *c, const double *radtemp, double lbol, double *out_opaclump,
+                    double *out_emitprob, double *out_emit, double *out_sthermal)
+{
+    if (!c || !radtemp) return fail(SLGPU_EINVAL, "null argument");
+    if (!c->have_opacity || !c->have_grid) return fail(SLGPU_EORDER, "opacity and grid must be set first");
+    const int nc = c->nx;
+    for (int i = 0; i < nc; ++i)
+        if (!(radtemp[i] > 0.0)) return fail(SLGPU_EINVAL, "grd_radtemp must be positive");
+    // SOURCE/sourceenergy.f90:23-50 (surface-like thermal source)
+    std::vector<double> emit(nc);
+    double s = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        const double t2 = radtemp[i] * radtemp[i];
+        emit[i] = PC_PI * PC_C * (c->h_rcell[i] * c->h_rcell[i]) * PC_ACOEF * (t2 * t2);
+        s += emit[i];
+    }
+    double st = 0.0;
+    for (int i = 0; i < nc; ++i) { emit[i] = emit[i] / s * lbol; st += emit[i]; }
+    PhaseTimer tm(c, 1);
+    HIPCHK(hipMemcpyAsync(c->d.emit, emit.data(), nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.radtemp, radtemp, nc * 8, hipMemcpyHostToDevice, c->stream));
+    launch_ddmc_tables(c);
+    HIPCHK(hipGetLastError());
+    if (out_opaclump) HIPCHK(hipMemcpyAsync(out_opaclump, c->d.opaclump, (size_t)11 * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_emitprob) HIPCHK(hipMemcpyAsync(out_emitprob, c->d.emitprob, (size_t)c->cfg.nep * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    int rc = tm.stop();
+    if (rc) return rc;
+    if (out_emit) std::memcpy(out_emit, emit.data(), nc * 8);
+    if (out_sthermal) *out_sthermal = st;
+    c->h_emit_cache = emit;
+    c->have_prep = true;
+    return 0;
+}
+
+int slgpu_sources(slgpu_ctx *c, int64_t nstot, uint32_t iteration, int *out_nvol,
+                  int64_t *out_nnew)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_prep) return fail(SLGPU_EORDER, "slgpu_ddmc_prep must run first");
+    if (nstot < 1) return fail(SLGPU_EINVAL, "nstot must be positive");
+    std::vector<int> nvol;
+    int64_t nnew = source_numbers(c->h_emit_cache, nstot, nvol);
+    if (nnew == -1) return fail(SLGPU_EINVAL, "sourcenumbers: no cell emits");
+    if (nnew == -2) return fail(SLGPU_EINVAL, "sourcenumbers: nextra>grd_ncell (sourcenumbers.f90:130-131)");
+    if (nnew > c->cfg.npartmax) return fail(SLGPU_EINVAL, "vacancies: nnew>npartmax (vacancies.f90:21)");
+    std::vector<int64_t> pstart(c->nx + 1, 0);
+    for (int i = 0; i < c->nx; ++i) pstart[i + 1] = pstart[i] + nvol[i];
+    PhaseTimer tm(c, 2);
+    HIPCHK(hipMemcpyAsync((void *)c->d.pstart, pstart.data(), (c->nx + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.flags, 0, 4, c->stream));
+    c->d.npart = nnew;
+    launch_sources(c, iteration);
+    HIPCHK(hipGetLastError());
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, c->d.flags, 4, hipMemcpyDeviceToHost, c->stream));
+    int rc = tm.stop();
+    if (rc) return rc;
+    if (flag) return fail(SLGPU_EINVAL, "interior_source: position rejection did not terminate");
+    if (out_nvol) std::memcpy(out_nvol, nvol.data(), c->nx * sizeof(int));
+    if (out_nnew) *out_nnew = nnew;
+    c->have_sources = true;
+    c->have_transport = false;
+    return 0;
+}
+
+int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double *out_jrad,
+                    double *out_totals, int64_t *out_counters)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_sources) return fail(SLGPU_EORDER, "slgpu_sources must run first");
+    const int nc = c->nx, ng = c->ng;
+    PhaseTimer tm(c, 3);
+    // particle_advance.f90:65-67: zero the tallies
+    HIPCHK(hipMemsetAsync(c->d.tally, 0, nc * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.jrad, 0, (size_t)nc * ng * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.tot, 0, 8 * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.ct, 0, 10 * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.queue, 0, 8, c->stream));
+    hipEventRecord(c->ev[12], c->stream);
+    launch_transport(c, iteration);
+    hipEventRecord(c->ev[13], c->stream);
+    HIPCHK(hipGetLastError());
+    double tot[8];
+    unsigned long long ct[10];
+    HIPCHK(hipMemcpyAsync(tot, c->d.tot, 8 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ct, c->d.ct, 10 * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_tally) HIPCHK(hipMemcpyAsync(out_tally, c->d.tally, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_jrad) HIPCHK(hipMemcpyAsync(out_jrad, c->d.jrad, (size_t)nc * ng * 8, hipMemcpyDeviceToHost, c->stream));
+    int rc = tm.stop();
+    if (rc) return rc;
+    float kms = 0.f;
+    hipEventElapsedTime(&kms, c->ev[12], c->ev[13]);
+    c->t_ms[6] += kms;
+    tot[2] = -tot[0];     // tot_sflux = -sum e(flux) (particle_advance.f90:283)
+    if (out_totals) {
+        const int map[6] = {0, 1, 2, 3, 4, 5};
+        for (int k = 0; k < 6; ++k) out_totals[k] = tot[map[k]];
+    }
+    if (out_counters)
+        for (int k = 0; k < 10; ++k) out_counters[k] = (int64_t)ct[k];
+    c->have_transport = true;
+    if (ct[7] > 0 && c->cfg.errorfatal)
+        return fail(SLGPU_ETRANSPORT, "particle_advance: fatal transport error (" + std::to_string(ct[7]) + " particles)");
+    return 0;
+}
+
+int slgpu_fluxtally(slgpu_ctx *c, double *out_luminos, double *out_lumdev, int *out_lumnum)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_transport) return fail(SLGPU_EORDER, "slgpu_transport must run first");
+    const int ng = c->ng;
+    PhaseTimer tm(c, 4);
+    HIPCHK(hipMemsetAsync(c->d.lum, 0, ng * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.dev, 0, ng * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.num, 0, ng * 4, c->stream));
+    launch_fluxtally(c);
+    HIPCHK(hipGetLastError());
+    if (out_luminos) HIPCHK(hipMemcpyAsync(out_luminos, c->d.lum, ng * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_lumdev) HIPCHK(hipMemcpyAsync(out_lumdev, c->d.dev, ng * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_lumnum) HIPCHK(hipMemcpyAsync(out_lumnum, c->d.num, ng * 4, hipMemcpyDeviceToHost, c->stream));
+    return tm.stop();
+}
+
+int slgpu_get_particles(slgpu_ctx *c, int64_t n, uint8_t *stat, double *x, double *mu,
+                        double *wl, double *e, int *ig, int64_t *nstep, int *err)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (n < 0 || n > c->d.npart) return fail(SLGPU_EINVAL, "n exceeds the particle count");
+    if (stat) HIPCHK(hipMemcpyAsync(stat, c->d.pstat, n, hipMemcpyDeviceToHost, c->stream));
+    if (x) HIPCHK(hipMemcpyAsync(x, c->d.px, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (mu) HIPCHK(hipMemcpyAsync(mu, c->d.pmu, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (wl) HIPCHK(hipMemcpyAsync(wl, c->d.pwl, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (e) HIPCHK(hipMemcpyAsync(e, c->d.pe, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (ig) HIPCHK(hipMemcpyAsync(ig, c->d.pig, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (nstep) HIPCHK(hipMemcpyAsync(nstep, c->d.pnstep, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (err) HIPCHK(hipMemcpyAsync(err, c->d.perr, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int slgpu_timers(slgpu_ctx *c, double *ms, int n, int reset)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    for (int k = 0; k < n && k < 8; ++k) ms[k] = c->t_ms[k];
+    if (reset)
+        for (double &t : c->t_ms) t = 0.0;
+    return 0;
+}
+
+}  // extern "C"

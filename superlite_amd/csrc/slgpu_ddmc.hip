@@ -1,0 +1,349 @@
+// DDMC preparation on gfx950: lumped leakage opacities (GRID/leakage_opacity11.f90:30-171),
+// binned emission probabilities (GRID/emission_probability.f90:39-62), and the per-cell
+// sampling tables that replace the reference's per-event O(ng) linear scans
+// (GRID/emitgroup.f:23-51, TRANSPORT1/diffusion11.f90:289-313, 375-398, 447-460,
+// 489-511, 571-584; SOURCE/interior_source.f90:96-103) by binary searches in the
+// transport and source kernels.  One 1024-thread block per cell; groups are swept in
+// chunks of 1024 with block-wide (wave64 shuffle) scans carrying the running prefix.
+#include "slgpu_device.h"
+#include "slgpu_internal.h"
+
+namespace slgpu {
+
+constexpr int NT = 1024;
+constexpr int NW = NT / 64;
+
+// inclusive block scan (fixed order) returning the block total through `total`
+__device__ __forceinline__ double block_scan(double v, double *sh, double &total)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double x = wave_inclusive_scan(v);
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        double t = lane < NW ? sh[lane] : 0.0;
+        t = wave_inclusive_scan(t);
+        if (lane < NW) sh[lane] = t;
+    }
+    __syncthreads();
+    double off = w > 0 ? sh[w - 1] : 0.0;
+    total = sh[NW - 1];
+    __syncthreads();
+    return x + off;
+}
+
+__device__ __forceinline__ int block_scan_int(int v, int *sh, int &total)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int x = v;
+    #pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(x, off, 64);
+        if (lane >= off) x += t;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int t = lane < NW ? sh[lane] : 0;
+        #pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int u = __shfl_up(t, off, 64);
+            if (lane >= off) t += u;
+        }
+        if (lane < NW) sh[lane] = t;
+    }
+    __syncthreads();
+    int off = w > 0 ? sh[w - 1] : 0;
+    total = sh[NW - 1];
+    __syncthreads();
+    return x + off;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    double t;
+    block_scan(v, sh, t);
+    return t;
+}
+
+#define CAPC(ig, ic) ((double)d.cap32[(size_t)((ic) - 1) * ng + ((ig) - 1)])
+#define XA(i) (d.xarr[(i) - 1])
+#define DX(i) (d.xarr[(i)] - d.xarr[(i) - 1])
+#define DX3(i) (d.xarr[(i)] * d.xarr[(i)] * d.xarr[(i)] - d.xarr[(i) - 1] * d.xarr[(i) - 1] * d.xarr[(i) - 1])
+#define SIGC(ic) (d.sig[(ic) - 1])
+
+// resolved leakage opacity of group g through the left (side<0) or right face of cell ix
+// (diffusion11.f90:204-239, 383-395, 455-457, 496-508)
+__device__ __forceinline__ double resleak(const Dev &d, int ix, int g, int side)
+{
+    const int ng = d.ng, ic = ix;
+    if (side < 0) {
+        const int l = ix - 1;
+        if ((CAPC(g, l) + SIGC(l)) * DX(ix - 1) < d.tauddmc) {
+            double mfp = (CAPC(g, ic) + SIGC(ic)) * DX(ix);
+            double ppl = 4.0 / (3.0 * mfp + 6.0 * PC_DEXT);
+            return 1.5 * ppl * (XA(ix) * XA(ix)) / DX3(ix);
+        }
+        double mfp = (SIGC(ic) + CAPC(g, ic)) * DX(ix) + (SIGC(l) + CAPC(g, l)) * DX(ix - 1);
+        return 2.0 * (XA(ix) * XA(ix)) / (mfp * DX3(ix));
+    }
+    if (ix == d.nx) {
+        double mfp = (CAPC(g, ic) + SIGC(ic)) * DX(ix);
+        double ppr = 4.0 / (3.0 * mfp + 6.0 * PC_DEXT);
+        return 1.5 * ppr * (XA(ix + 1) * XA(ix + 1)) / DX3(ix);
+    }
+    const int l = ix + 1;
+    if ((CAPC(g, l) + SIGC(l)) * DX(ix + 1) < d.tauddmc) {
+        double mfp = (CAPC(g, ic) + SIGC(ic)) * DX(ix);
+        double ppr = 4.0 / (3.0 * mfp + 6.0 * PC_DEXT);
+        return 1.5 * ppr * (XA(ix + 1) * XA(ix + 1)) / DX3(ix);
+    }
+    double mfp = (SIGC(ic) + CAPC(g, ic)) * DX(ix) + (SIGC(l) + CAPC(g, l)) * DX(ix + 1);
+    return 2.0 * (XA(ix + 1) * XA(ix + 1)) / (mfp * DX3(ix));
+}
+
+__global__ __launch_bounds__(NT) void k_leakage(Dev d)
+{
+    __shared__ double shd[NW];
+    __shared__ int shi[NW];
+    __shared__ double s_emax[NW];
+    __shared__ int s_imax[NW];
+    const int ng = d.ng, nx = d.nx;
+    const int i = blockIdx.x + 1, l = i;
+    const int icnb1 = i - 1 < 1 ? 1 : i - 1, icnb2 = i + 1 > nx ? nx : i + 1;
+    const double dist = DX(i);
+    const double tinv = d.tempinv[l - 1];
+    const double sg = SIGC(l);
+    const double *spec = d.specarr + (size_t)(l - 1) * ng;
+    const double divv = d.divv[l - 1];
+    // pass 1: speclump and the lump size (leakage_opacity11.f90:52-59)
+    double sl = 0.0;
+    int nl = 0;
+    for (int g = threadIdx.x + 1; g <= ng; g += NT) {
+        const double c = CAPC(g, l);
+        if (c * dist >= d.taulump && (sg + c) * dist >= d.tauddmc) { sl += spec[g - 1]; nl++; }
+    }
+    sl = block_sum(sl, shd);
+    int nlump;
+    block_scan_int(nl, shi, nlump);
+    const double speclump = sl > 0.0 ? 1.0 / sl : 0.0;
+    // pass 2: caplump, igemitmax, doplump, leakage opacities (:61-168)
+    double caplump = 0.0, doplump = 0.0, o1 = 0.0, o2 = 0.0, emax = 0.0;
+    int imax = 0;
+    const double dvol = 3.0 * d.vol[l - 1] / PC_PI4;
+    for (int g = threadIdx.x + 1; g <= ng; g += NT) {
+        const double c = CAPC(g, l);
+        const bool lump = !(c * dist < d.taulump) && !((sg + c) * dist < d.tauddmc);
+        if (lump) {
+            const double h = spec[g - 1] * c;
+            caplump += h;
+            if (h > emax) { emax = h; imax = g; }
+            const double specval = spec[g - 1];
+            bool lh = (i == 1) ? true : ((CAPC(g, icnb1) + SIGC(icnb1)) * DX(i - 1) < d.tauddmc);
+            if (lh) {
+                double help = (c + sg) * DX(i);
+                double ppl = 4.0 / (3.0 * help + 6.0 * PC_DEXT);
+                o1 += (specval * speclump) * 1.5 * ppl * (XA(i) * XA(i)) / dvol;
+            } else {
+                double help = (sg + c) * DX(i) + (SIGC(icnb1) + CAPC(g, icnb1)) * DX(i - 1);
+                o1 += (specval * speclump) * 2.0 * (XA(i) * XA(i)) / (help * 3.0 * d.vol[l - 1] / PC_PI4);
+            }
+            lh = (i == nx) ? true : ((CAPC(g, icnb2) + SIGC(icnb2)) * DX(i + 1) < d.tauddmc);
+            if (lh) {
+                double help = (c + sg) * DX(i);
+                double ppr = 4.0 / (3.0 * help + 6.0 * PC_DEXT);
+                o2 += (specval * speclump) * 1.5 * ppr * (XA(i + 1) * XA(i + 1)) / dvol;
+            } else {
+                double help = (sg + c) * DX(i) + (SIGC(icnb2) + CAPC(g, icnb2)) * DX(i + 1);
+                o2 += (specval * speclump) * 2.0 * (XA(i + 1) * XA(i + 1)) / (help * 3.0 * d.vol[l - 1] / PC_PI4);
+            }
+            if (divv >= 0) {
+                if (g <= ng - 1 && !(CAPC(g + 1, l) * dist >= d.taulump))
+                    doplump += dopspeccalc(tinv, d.wlinv, g) * divv / (3 * PC_C);
+            } else {
+                if (g >= 2 && !(CAPC(g - 1, l) * dist >= d.taulump))
+                    doplump -= dopspeccalc(tinv, d.wlinv, g - 1) * divv / (3 * PC_C);
+            }
+        }
+    }
+    caplump = block_sum(caplump, shd);
+    doplump = block_sum(doplump, shd);
+    o1 = block_sum(o1, shd);
+    o2 = block_sum(o2, shd);
+    // first index of the maximum (strict '>' in the reference's ascending loop)
+    {
+        double v = emax;
+        int k = imax;
+        for (int off = 32; off > 0; off >>= 1) {
+            double v2 = __shfl_xor(v, off, 64);
+            int k2 = __shfl_xor(k, off, 64);
+            if (v2 > v || (v2 == v && k2 != 0 && (k == 0 || k2 < k))) { v = v2; k = k2; }
+        }
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { s_emax[w] = v; s_imax[w] = k; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double bv = 0.0;
+            int bk = 0;
+            for (int q = 0; q < NW; ++q)
+                if (s_emax[q] > bv || (s_emax[q] == bv && s_imax[q] != 0 && (bk == 0 || s_imax[q] < bk))) {
+                    bv = s_emax[q]; bk = s_imax[q];
+                }
+            imax = bk;
+        }
+    }
+    double *op = d.opaclump + (size_t)(l - 1) * 11;
+    const double capgrey = d.capgrey[l - 1];
+    double cgi = 1.0 / capgrey;
+    if (!(cgi >= 0.0)) cgi = 0.0;
+    const int nl_eff = speclump == 0.0 ? 0 : nlump;
+    double c_emitlump, c_caplump, c_doplump;
+    if (nl_eff == ng) {
+        c_emitlump = 1.0; c_caplump = capgrey; c_doplump = 0.0;
+    } else {
+        c_caplump = caplump * speclump;
+        double el = caplump * cgi;
+        c_emitlump = el < 1.0 ? el : 1.0;
+        c_doplump = doplump * speclump;
+    }
+    if (threadIdx.x == 0) {
+        op[0] = o1; op[1] = o2; op[2] = 0.0; op[3] = 0.0; op[4] = 0.0; op[5] = 0.0;
+        op[6] = speclump; op[7] = caplump; op[8] = 0.0; op[9] = (double)imax; op[10] = doplump;
+        CellDDMC cc;
+        cc.capgreyinv = cgi; cc.speclump = speclump; cc.emitlump = c_emitlump;
+        cc.caplump = c_caplump; cc.doplump = c_doplump; cc.nlump = nl_eff; cc.pad = 0;
+        d.cell[l - 1] = cc;
+    }
+    if (nl_eff == 0) return;
+
+    // pass 3: lump order and cumulative sampling weights
+    int *gorder = d.gorder + (size_t)(l - 1) * ng;
+    int baseL = 0, baseU = nl_eff;
+    for (int g0 = 1; g0 <= ng; g0 += NT) {
+        const int g = g0 + threadIdx.x;
+        int lump = 0;
+        if (g <= ng) {
+            const double c = CAPC(g, l);
+            lump = (c * dist >= d.taulump && (sg + c) * dist >= d.tauddmc) ? 1 : 0;
+        }
+        int tl, tu;
+        int pl = block_scan_int(lump, shi, tl);
+        int pu = block_scan_int(g <= ng ? 1 - lump : 0, shi, tu);
+        if (g <= ng) gorder[lump ? baseL + pl - 1 : baseU + pu - 1] = g;
+        baseL += tl;
+        baseU += tu;
+    }
+    __syncthreads();
+    const double hL = 1.0 / o1, hR = 1.0 / o2, hD = 1.0 / c_doplump;
+    const double d2s = 1.0 / (1.0 - c_emitlump);
+    double *cL = d.cumL + (size_t)(l - 1) * ng, *cR = d.cumR + (size_t)(l - 1) * ng;
+    double *cD = d.cumD + (size_t)(l - 1) * ng, *cS = d.cumS + (size_t)(l - 1) * ng;
+    double carL = 0.0, carR = 0.0, carD = 0.0;
+    for (int p0 = 0; p0 < nl_eff; p0 += NT) {
+        const int p = p0 + threadIdx.x;
+        double tL = 0.0, tR = 0.0, tD = 0.0;
+        if (p < nl_eff) {
+            // the Doppler scan runs ascending for redshift, descending for blueshift
+            const int pd = divv >= 0 ? p : nl_eff - 1 - p;
+            const int g = gorder[p], gd = gorder[pd];
+            const double sp = spec[g - 1];
+            if (i > 1) tL = sp * resleak(d, i, g, -1) * speclump * hL;
+            tR = sp * resleak(d, i, g, +1) * speclump * hR;
+            if (divv >= 0) {
+                if (gd != ng && !(CAPC(gd + 1, l) * dist >= d.taulump))
+                    tD = dopspeccalc(tinv, d.wlinv, gd) * divv / (3 * PC_C) * speclump * hD;
+            } else {
+                if (gd != 1 && !(CAPC(gd - 1, l) * dist >= d.taulump))
+                    tD = -(dopspeccalc(tinv, d.wlinv, gd - 1) * divv / (3 * PC_C) * speclump * hD);
+            }
+        }
+        double tot;
+        double sL = block_scan(tL, shd, tot); if (p < nl_eff) cL[p] = carL + sL; carL += tot;
+        double sR = block_scan(tR, shd, tot); if (p < nl_eff) cR[p] = carR + sR; carR += tot;
+        double sD = block_scan(tD, shd, tot); if (p < nl_eff) cD[p] = carD + sD; carD += tot;
+    }
+    double carS = 0.0;
+    for (int p0 = nl_eff; p0 < ng; p0 += NT) {
+        const int p = p0 + threadIdx.x;
+        double tS = 0.0;
+        if (p < ng) {
+            const int g = gorder[p];
+            tS = spec[g - 1] * CAPC(g, l) * cgi * d2s;
+        }
+        double tot;
+        double sS = block_scan(tS, shd, tot);
+        if (p < ng) cS[p] = carS + sS;
+        carS += tot;
+    }
+}
+
+// emission_probability.f90:39-62 and the in-bin cumulative sums emitgroup.f:278-294 walks
+__global__ __launch_bounds__(256) void k_emitprob(Dev d)
+{
+    __shared__ double binsum[1024];
+    __shared__ double pref[1025];
+    const int ng = d.ng, nep = d.nep, nepg = d.nepg;
+    const int l = blockIdx.x + 1;
+    const double *spec = d.specarr + (size_t)(l - 1) * ng;
+    for (int b = threadIdx.x; b < nep; b += 256) {
+        const int g0 = b * nepg + 1;
+        const int g1 = (b + 1) * nepg < ng ? (b + 1) * nepg : ng;
+        double s = 0.0;
+        for (int g = g0; g <= g1; ++g) s += spec[g - 1] * CAPC(g, l);
+        binsum[b] = s;
+    }
+    __syncthreads();
+    double *ep = d.emitprob + (size_t)(l - 1) * nep;
+    if (threadIdx.x == 0) {
+        double help = 0.0;
+        pref[0] = 0.0;
+        for (int iep = 1; iep <= nep - 1; ++iep) {
+            help = help + binsum[iep - 1];
+            ep[iep - 1] = help;
+            pref[iep] = help;
+        }
+        ep[nep - 1] = d.capgrey[l - 1];
+    }
+    __syncthreads();
+    double *cum = d.emitcum + (size_t)(l - 1) * ng;
+    for (int b = threadIdx.x; b < nep; b += 256) {
+        const int g0 = b * nepg + 1;
+        const int g1 = (b + 1) * nepg < ng ? (b + 1) * nepg : ng;
+        double s = pref[b];
+        for (int g = g0; g <= g1; ++g) {
+            s = s + spec[g - 1] * CAPC(g, l);
+            cum[g - 1] = s;
+        }
+    }
+}
+
+// normalised Planck at T_rad and its cumulative (interior_source.f90:69-75, 96-103)
+__global__ __launch_bounds__(NT) void k_srccum(Dev d)
+{
+    __shared__ double shd[NW];
+    const int ng = d.ng;
+    const int l = blockIdx.x;
+    const double tr = 1.0 / d.radtemp[l];
+    double s = 0.0;
+    for (int g = threadIdx.x + 1; g <= ng; g += NT) s += specintv_g(tr, d.wlinv, g);
+    s = block_sum(s, shd);
+    double *cum = d.srccum + (size_t)l * ng;
+    double car = 0.0;
+    for (int g0 = 1; g0 <= ng; g0 += NT) {
+        const int g = g0 + threadIdx.x;
+        const double p = g <= ng ? specintv_g(tr, d.wlinv, g) / s : 0.0;
+        double tot;
+        double x = block_scan(p, shd, tot);
+        if (g <= ng) cum[g - 1] = car + x;
+        car += tot;
+    }
+}
+
+void launch_ddmc_tables(slgpu_ctx *c)
+{
+    hipLaunchKernelGGL(k_leakage, dim3(c->nx), dim3(NT), 0, c->stream, c->d);
+    hipLaunchKernelGGL(k_emitprob, dim3(c->nx), dim3(256), 0, c->stream, c->d);
+    hipLaunchKernelGGL(k_srccum, dim3(c->nx), dim3(NT), 0, c->stream, c->d);
+}
+
+}  // namespace slgpu

@@ -1,0 +1,122 @@
+// Internal state of an slgpu context and the kernel launchers shared between the
+// translation units of libslgpu.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/slgpu.h"
+
+namespace slgpu {
+
+// One packed line record (the bb_xs fields the binning needs), 16 bytes.
+struct LineRec {
+    float gxs;      // g*f*(pi e^2/m_e c)
+    float chilw;    // lower-level energy [1/cm]
+    int ion;        // index into the population list
+    int pad;
+};
+
+// Per-cell DDMC constants (diffusion11.f90:95-152), evaluated once per iteration.
+struct CellDDMC {
+    double capgreyinv, speclump, emitlump, caplump, doplump;
+    int nlump;
+    int pad;
+};
+
+// Everything the kernels read or write, passed to kernels by value.
+struct Dev {
+    int nx, ng, nep, nepg, nion, nel;
+    double tauddmc, taulump;
+    int puretran, precision;
+    int64_t max_steps;
+    uint32_t seed;
+    // mesh (nx+1 / nx)
+    const double *xarr, *vxarr, *divv, *vol;
+    // groups (ng+1)
+    const double *wl, *wlinv;
+    // lines
+    const LineRec *lines;
+    const int *gstart;          // ng+2: lines of group g (1-based) are [gstart[g], gstart[g+1])
+    // tables
+    double *bfxs;               // ng*nion, xs(ig, k)*1e-18 (Mb -> cm^2)
+    const double *gff;          // 81*41 Fortran order
+    // per-iteration gas state (cell-fastest, [k*nc + i])
+    double *hckt, *hlparr, *ocg, *ocT, *sig, *tempinv, *temp, *mass, *natomfr;
+    int *el_z;                  // nel present elements (ascending iz)
+    // opacity products
+    double *capT;               // ng*nc cell-fastest scratch
+    double *cap64;              // nc*ng group-fastest
+    float *cap32, *emiss;       // nc*ng group-fastest (gas_cap, gas_emiss)
+    double *capgrey, *capross;
+    // DDMC prep
+    double *specarr;            // nc*ng  specintv(tempinv(ic)) group-fastest
+    double *opaclump;           // 11*nc
+    double *emitprob;           // nep*nc
+    double *emitcum;            // nc*ng  cumulative emission probability inside each bin
+    double *emit;               // nc
+    double *radtemp;            // nc
+    double *srccum;             // nc*ng  cumulative normalised Planck at T_rad
+    CellDDMC *cell;             // nc
+    int *gorder;                // nc*ng  lumped groups ascending, then unlumped ascending
+    double *cumL, *cumR, *cumD, *cumS;   // nc*ng cumulative lumped sampling weights
+    // particles (SoA)
+    int64_t npart;
+    const int64_t *pstart;      // nc+1
+    double *px, *pmu, *pwl, *pe, *pe0;
+    uint8_t *pstat;
+    int *pig, *perr;
+    int64_t *pnstep;
+    // tallies
+    double *tally, *jrad, *tot;           // nc, nc*ng, 8
+    unsigned long long *ct;               // 10
+    unsigned long long *queue;            // work counter of the persistent transport
+    double *lum, *dev;                    // ng
+    int *num;                             // ng
+    int *flags;                           // bad input flags raised by kernels
+};
+
+struct Timer {
+    hipEvent_t a, b;
+    bool armed = false;
+};
+
+}  // namespace slgpu
+
+struct slgpu_ctx {
+    slgpu_config cfg;
+    slgpu::Dev d;
+    hipStream_t stream = nullptr;
+    int ng = 0, nx = 0, nion = 0;
+    bool have_grid = false, have_groups = false, have_lines = false, have_bf = false,
+         have_ff = false, have_ions = false, have_opacity = false, have_prep = false,
+         have_sources = false, have_transport = false;
+    int64_t nline = 0, nline_used = 0;
+    std::vector<double> h_xarr, h_vxarr, h_divv, h_vol, h_rcell, h_wl;
+    std::vector<int> h_ion_iz, h_ion_ii;
+    std::vector<float> h_ph1, h_ph2;
+    std::vector<float> h_wl0;
+    std::vector<int16_t> h_liz, h_lii;
+    std::vector<float> h_gxs, h_chilw;
+    std::vector<double> h_emit_cache;
+    std::vector<void *> allocations;
+    double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[16];
+};
+
+namespace slgpu {
+// launchers (each enqueues on ctx->stream)
+void launch_bfxs(slgpu_ctx *c, const float *d_ph1, const float *d_ph2, const int *d_iz,
+                 const int *d_ii);
+void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
+                      const double *d_nelec, const double *d_oc, const double *d_ginv,
+                      int flags);
+void launch_opacity(slgpu_ctx *c, int flags);
+void launch_opacity_finish(slgpu_ctx *c);
+void launch_ddmc_tables(slgpu_ctx *c);
+void launch_sources(slgpu_ctx *c, uint32_t iteration);
+void launch_transport(slgpu_ctx *c, uint32_t iteration);
+void launch_fluxtally(slgpu_ctx *c);
+}  // namespace slgpu

@@ -1,0 +1,294 @@
+// LTE group-opacity build on gfx950 (GAS/physical_opacity.f:3-357, grp_ngs == 1) and the
+// Planck / Rosseland means (GAS/opacity_planckmean.f:18-24, GAS/opacity_rossmean.f:18-21).
+//
+// Work decomposition.  One thread per (cell, group).  A wave holds 64 consecutive cells
+// of ONE group, so the group's slice of the lambda-sorted line list is wave-uniform: the
+// line records stream through the scalar unit (s_load) while the 64 lanes evaluate the
+// per-cell Boltzmann factor exp(-chi_low*hc/kT) in parallel.  Each (cell, group) sum is a
+// segment of the segmented reduction over the sorted lines; the segment boundaries are
+// the precomputed per-group line offsets, so every sum runs in the reference's line order
+// and no cross-lane combine is needed.  bf and ff are accumulated into the same register
+// afterwards, in the reference's order (bb, then bf by ion, then ff by element).
+#include "slgpu_device.h"
+#include "slgpu_internal.h"
+
+namespace slgpu {
+
+#define PH1(p, s, z, e) ((double)ph1[(p - 1) + 6 * ((s - 1) + 7 * ((z - 1) + 30 * (e - 1)))])
+#define PH2(p, z, e) ((double)ph2[(p - 1) + 7 * ((z - 1) + 30 * (e - 1))])
+
+// bfxsmod.f:65-153: Verner 1995/96 partial photo-ionisation cross sections [Mb]
+__device__ double bfxs_dev(int nz, int ne, double e, const float *__restrict__ ph1,
+                           const float *__restrict__ ph2)
+{
+    const int ll[7] = {0, 0, 1, 0, 1, 2, 0};
+    const int ninn[30] = {0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 3, 3, 3, 3,
+                          5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5};
+    const int ntot[30] = {1, 1, 2, 2, 3, 3, 3, 3, 3, 3, 4, 4, 5, 5, 5, 5, 5, 5,
+                          6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 7, 7};
+    double xs = 0.0;
+    if (nz < 1 || nz > 30) return xs;
+    if (ne < 1 || ne > nz) return xs;
+    int nout = ntot[ne - 1];
+    if (nz > 18) {
+        if (nz == ne) nout = 7;
+        if (nz == ne + 1 && (nz == 20 || nz == 21 || nz == 22 || nz == 25 || nz == 26)) nout = 7;
+    }
+    if (e < PH1(1, nout, nz, ne)) return xs;
+    const int nin = ninn[ne - 1];
+    double einn;
+    if (nz == 15 || nz == 17 || nz == 19 || (nz > 20 && nz != 26)) einn = 0.0;
+    else einn = (ne < 3) ? (double)1.0e30f : PH1(1, nin, nz, ne);
+    for (int is = 1; is <= nout; ++is) {
+        if (e < PH1(1, is, nz, ne)) continue;
+        if (is < nout && is > nin && e < einn) continue;
+        if (is <= nin || e >= einn) {
+            double p1 = -PH1(5, is, nz, ne);
+            double y = e / PH1(2, is, nz, ne);
+            double q = -0.5 * p1 - ll[is - 1] - 5.5;
+            double a = PH1(3, is, nz, ne) * ((y - 1.0) * (y - 1.0) + PH1(6, is, nz, ne) * PH1(6, is, nz, ne));
+            double b = sqrt(y / PH1(4, is, nz, ne)) + 1.0;
+            xs = xs + a * pow(y, q) * pow(b, p1);
+        } else {
+            double p1 = -PH2(4, nz, ne);
+            double q = -0.5 * p1 - 5.5;
+            double x = e / PH2(1, nz, ne) - PH2(6, nz, ne);
+            double z = sqrt(x * x + PH2(7, nz, ne) * PH2(7, nz, ne));
+            double a = PH2(2, nz, ne) * ((x - 1.0) * (x - 1.0) + PH2(5, nz, ne) * PH2(5, nz, ne));
+            double b = 1.0 + sqrt(z / PH2(3, nz, ne));
+            xs = xs + a * pow(z, q) * pow(b, p1);
+        }
+    }
+    return xs;
+}
+
+// bf cross sections are cell-independent: xs(ig, ion) * pc_mbarn at the left group edge
+// (physical_opacity.f:177-187), computed once per group grid / table upload.
+__global__ void k_bfxs(Dev d, const float *__restrict__ ph1, const float *__restrict__ ph2,
+                       const int *__restrict__ iz, const int *__restrict__ ii)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.ng * d.nion) return;
+    int g = t / d.nion, k = t - g * d.nion;
+    double wl = d.wl[g];
+    double en = PC_H * PC_C / (PC_EV * wl);
+    int z = iz[k];
+    double xs = bfxs_dev(z, z - ii[k] + 1, en, ph1, ph2);
+    d.bfxs[t] = xs * PC_MBARN;
+}
+
+// per-cell helpers (physical_opacity.f:81-89, 98-100, 168-169, 203) and the ff gamma^2
+// interpolation coordinate per (element, cell) (physical_opacity.f:226-227)
+__global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const double *__restrict__ vol,
+                            const double *__restrict__ nelec, const double *__restrict__ oc,
+                            const double *__restrict__ ginv, int flags, double *rgg_out)
+{
+    const int nc = d.nx;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int nk = d.nion > d.nel ? d.nion : d.nel;
+    if (nk < 1) nk = 1;
+    if (t >= nc * nk) return;
+    int k = t / nc, i = t - k * nc;
+    double hckt = PC_H * PC_C / (PC_KB * d.temp[i]);
+    if (k == 0) {
+        constexpr double cthomson = 8.0 * PC_PI * (PC_E * PC_E * PC_E * PC_E) /
+                                    (3.0 * (PC_ME * PC_ME) * (PC_C * PC_C * PC_C * PC_C));
+        d.hckt[i] = hckt;
+        d.tempinv[i] = 1.0 / d.temp[i];
+        double s = (flags & SLGPU_NOTHOMSON) ? 0.0 : cthomson * nelec[i] * natom[i] / vol[i];
+        if (d.mass[i] <= 0.0) s = 0.0;
+        d.sig[i] = s;
+        double nv = natom[i] / vol[i];
+        d.hlparr[i] = nv * nv * nelec[i];
+    }
+    if (k < d.nion) {
+        double o = oc[(size_t)i * d.nion + k];
+        d.ocT[(size_t)k * nc + i] = o;
+        d.ocg[(size_t)k * nc + i] = o * ginv[(size_t)i * d.nion + k];
+    }
+    if (k < d.nel) {
+        int z = d.el_z[k];
+        double gg = (double)(z * z) * PC_RYDBERG * hckt;
+        rgg_out[(size_t)k * nc + i] = 5.0 * (log10(gg) + 4.0) + 1.0;
+    }
+}
+
+// Fused bb + bf + ff for one (cell, group); result into capT[g*nc + i] (cell-fastest).
+// blockDim = 256 = 64 cells x 4 groups; grid = (ceil(nc/64), ceil(ng/4)).
+__global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
+                                                  const double *__restrict__ rgg_tab)
+{
+    const int nc = d.nx, ng = d.ng;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
+    // wave-uniform group index (1-based)
+    const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6)) + 1;
+    if (g > ng) return;
+    const bool act = i < nc;
+    const int ic = act ? i : nc - 1;
+    const bool massive = act && d.mass[ic] > 0.0;
+    const double hckt = d.hckt[ic];
+    const double wlo = d.wl[g - 1], whi = d.wl[g];
+    double acc = 0.0;
+
+    if (!(flags & SLGPU_NOBB)) {
+        // physical_opacity.f:128-137: subgroup-centre profile and stimulated emission
+        const double wlc = 0.5 * (wlo + whi);
+        const double wlinv = 1.0 / wlc;
+        const double dwl = whi - wlo;
+        const double expfac = 1.0 - exp(-hckt * wlinv);
+        const double phi = wlc * wlc / (dwl * PC_C);
+        const int l0 = d.gstart[g], l1 = d.gstart[g + 1];
+        const double *ocg = d.ocg + ic;
+        for (int l = l0; l < l1; ++l) {
+            const LineRec r = d.lines[l];                 // uniform: scalar loads
+            const double o = ocg[(size_t)r.ion * nc];
+            const double term = phi * (double)r.gxs * o * exp(-((double)r.chilw * hckt)) * expfac;
+            acc += (o > 0.0) ? term : 0.0;
+        }
+    }
+    if (!(flags & SLGPU_NOBF)) {
+        // physical_opacity.f:177-189 (cross sections precomputed per group and ion)
+        const double *xs = d.bfxs + (size_t)(g - 1) * d.nion;
+        for (int k = 0; k < d.nion; ++k) {
+            const double x = xs[k];
+            if (x != 0.0) acc += x * d.ocT[(size_t)k * nc + ic];
+        }
+    }
+    if (!(flags & SLGPU_NOFF)) {
+        // physical_opacity.f:210-243 (left group edge)
+        const double wl = wlo;
+        const double u = hckt * (1.0 / wl);
+        const double ru = 10.0 * (log10(u) + 4.0) + 1.0;
+        int iu = (int)floor(ru);
+        iu = iu < 1 ? 1 : (iu > FF_NU - 1 ? FF_NU - 1 : iu);
+        const double du = ru - iu;
+        const double help = ffc1 * sqrt(hckt) * (1.0 - exp(-u)) * (wl * wl * wl) * d.hlparr[ic];
+        for (int k = 0; k < d.nel; ++k) {
+            const int z = d.el_z[k];
+            const double rgg = rgg_tab[(size_t)k * nc + ic];
+            int igg = (int)floor(rgg);
+            igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
+            const double dgg = rgg - igg;
+            const double *T = d.gff;
+            double gff = (1.0 - du) * (1.0 - dgg) * T[(igg - 1) * FF_NU + (iu - 1)] +
+                         du * (1.0 - dgg) * T[(igg - 1) * FF_NU + iu] +
+                         (1.0 - du) * dgg * T[igg * FF_NU + (iu - 1)] +
+                         du * dgg * T[igg * FF_NU + iu];
+            if (rgg > FF_NGG) gff = gff > 1.0 ? gff : 1.0;
+            acc += help * gff * (double)(z * z) * d.natomfr[(size_t)ic * 30 + (z - 1)];
+        }
+    }
+    if (act) d.capT[(size_t)(g - 1) * nc + i] = massive ? acc : 0.0;
+}
+
+// capT (ng, nc) cell-fastest -> cap64 / gas_cap / gas_emiss (nc, ng) group-fastest
+// (physical_opacity.f:267-279), 32x32 tiles through LDS.
+__global__ __launch_bounds__(256) void k_opacity_finish(Dev d, int want64)
+{
+    __shared__ double tile[32][33];
+    const int nc = d.nx, ng = d.ng;
+    const int g0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        int g = g0 + r, i = i0 + tx;
+        tile[r][tx] = (g < ng && i < nc) ? d.capT[(size_t)g * nc + i] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        int i = i0 + r, g = g0 + tx;
+        if (i < nc && g < ng) {
+            double c = tile[tx][r];
+            size_t o = (size_t)i * ng + g;
+            if (want64) d.cap64[o] = c;
+            d.cap32[o] = (float)c;
+            double em = 0.0;
+            if (d.mass[i] > 0.0) {
+                double wl = 0.5 * (d.wl[g + 1] + d.wl[g]);
+                em = c * ((wl * wl / PC_C) * planck(wl, d.temp[i]));
+            }
+            d.emiss[o] = (float)em;
+        }
+    }
+}
+
+// Block-wide deterministic sum (fixed tree) for 256-thread blocks.
+__device__ __forceinline__ double block_sum256(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// Planck group integrals specintv(1/T) for the DDMC/emission tables, and the Planck and
+// Rosseland means of the FP32 group opacity.  One 256-thread block per cell.
+__global__ __launch_bounds__(256) void k_cell_tables(Dev d)
+{
+    __shared__ double sh[4];
+    const int i = blockIdx.x, ng = d.ng;
+    const double tinv = d.tempinv[i];
+    const double sg = d.sig[i];
+    double pg = 0.0, pr = 0.0;
+    for (int g = threadIdx.x + 1; g <= ng; g += 256) {
+        const double c = (double)d.cap32[(size_t)i * ng + g - 1];
+        const double s = specintv_g(tinv, d.wlinv, g);
+        d.specarr[(size_t)i * ng + g - 1] = s;
+        pg += c * s;
+        pr += (1.0 / (sg + c)) * spectintv_g(tinv, d.wlinv, g);
+    }
+    pg = block_sum256(pg, sh);
+    pr = block_sum256(pr, sh);
+    if (threadIdx.x == 0) {
+        d.capgrey[i] = pg;
+        d.capross[i] = 1.0 / pr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+static double *g_rgg_scratch(slgpu_ctx *c)
+{
+    // per (element, cell) ff coordinate, lives in the ocT allocation's tail
+    return c->d.ocT + (size_t)c->nion * c->nx;
+}
+
+void launch_bfxs(slgpu_ctx *c, const float *d_ph1, const float *d_ph2, const int *d_iz,
+                 const int *d_ii)
+{
+    int n = c->ng * c->nion;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_bfxs, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d, d_ph1,
+                       d_ph2, d_iz, d_ii);
+}
+
+void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
+                      const double *d_nelec, const double *d_oc, const double *d_ginv, int flags)
+{
+    int nk = c->nion > c->d.nel ? c->nion : c->d.nel;
+    if (nk < 1) nk = 1;
+    int n = c->nx * nk;
+    hipLaunchKernelGGL(k_cell_prep, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d,
+                       d_natom, d_vol, d_nelec, d_oc, d_ginv, flags, g_rgg_scratch(c));
+}
+
+void launch_opacity(slgpu_ctx *c, int flags)
+{
+    const double ffc1 = 4.0 * (PC_E * PC_E * PC_E * PC_E * PC_E * PC_E) /
+                        (3.0 * PC_H * PC_ME * (PC_C * PC_C * PC_C * PC_C)) *
+                        std::sqrt(PC_PI2 / (3 * PC_ME * PC_H * PC_C));
+    dim3 grid((c->nx + 63) / 64, (c->ng + 3) / 4);
+    hipLaunchKernelGGL(k_opacity, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,
+                       (const double *)g_rgg_scratch(c));
+}
+
+void launch_opacity_finish(slgpu_ctx *c)
+{
+    dim3 grid((c->ng + 31) / 32, (c->nx + 31) / 32);
+    hipLaunchKernelGGL(k_opacity_finish, grid, dim3(256), 0, c->stream, c->d, 1);
+    hipLaunchKernelGGL(k_cell_tables, dim3(c->nx), dim3(256), 0, c->stream, c->d);
+}
+
+}  // namespace slgpu

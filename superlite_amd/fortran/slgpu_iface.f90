@@ -1,0 +1,168 @@
+! iso_c_binding interfaces to libslgpu.so (include/slgpu.h), for a SuperLite host.
+!
+! A patched SuperLite keeps its driver (superlite.f90) and module state and replaces the
+! bodies of physical_opacity + opacity_planckmean + opacity_rossmean, leakage_opacity +
+! emission_probability (+ sourceenergy), sourcenumbers/vacancies/interior_source/
+! source_transformdirection, particle_advance and fluxtally by calls below.
+module slgpu_iface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: SLGPU_OK = 0, SLGPU_EINVAL = 1, SLGPU_EORDER = 2, &
+       SLGPU_EHIP = 3, SLGPU_ETRANSPORT = 4, SLGPU_ENOMEM = 5
+  integer(c_int), parameter :: SLGPU_FP64 = 0, SLGPU_MIXED = 1
+  integer(c_int), parameter :: SLGPU_NOBB = 1, SLGPU_NOBF = 2, SLGPU_NOFF = 4, SLGPU_NOTHOMSON = 8
+
+  type, bind(C) :: slgpu_config
+    integer(c_int) :: nx, ng, nep, nepg
+    real(c_double) :: tauddmc, taulump
+    integer(c_int) :: puretran, errorfatal, precision
+    integer(c_int32_t) :: seed
+    integer(c_int64_t) :: npartmax, max_steps
+  end type
+
+  type, bind(C) :: slgpu_gas
+    type(c_ptr) :: temp, natom, vol, mass, nelec, natomfr, oc, ginv
+  end type
+
+  interface
+    integer(c_int) function slgpu_create(cfg, ctx) bind(C, name="slgpu_create")
+      import :: c_int, c_ptr, slgpu_config
+      type(slgpu_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function slgpu_destroy(ctx) bind(C, name="slgpu_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    type(c_ptr) function slgpu_last_error_message() bind(C, name="slgpu_last_error_message")
+      import :: c_ptr
+    end function
+
+    integer(c_int) function slgpu_set_grid(ctx, xarr, vxarr, divv, vol, rcell) &
+         bind(C, name="slgpu_set_grid")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: xarr(*), vxarr(*), divv(*), vol(*), rcell(*)
+    end function
+
+    integer(c_int) function slgpu_set_groups(ctx, grp_wl) bind(C, name="slgpu_set_groups")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: grp_wl(*)
+    end function
+
+    integer(c_int) function slgpu_set_lines(ctx, nline, wl0, gxs, chilw, iz, ii) &
+         bind(C, name="slgpu_set_lines")
+      import :: c_int, c_ptr, c_float, c_int16_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nline
+      real(c_float), intent(in) :: wl0(*), gxs(*), chilw(*)
+      integer(c_int16_t), intent(in) :: iz(*), ii(*)
+    end function
+
+    integer(c_int) function slgpu_set_bf_tables(ctx, ph1, ph2) bind(C, name="slgpu_set_bf_tables")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: ph1(*), ph2(*)
+    end function
+
+    integer(c_int) function slgpu_set_ff_table(ctx, gff) bind(C, name="slgpu_set_ff_table")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: gff(*)
+    end function
+
+    integer(c_int) function slgpu_set_ions(ctx, nion, ion_iz, ion_ii) bind(C, name="slgpu_set_ions")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nion
+      integer(c_int), intent(in) :: ion_iz(*), ion_ii(*)
+    end function
+
+    integer(c_int) function slgpu_opacity_build(ctx, gas, flags, cap64, cap32, sig, capgrey, &
+         capross, emiss) bind(C, name="slgpu_opacity_build")
+      import :: c_int, c_ptr, slgpu_gas
+      type(c_ptr), value :: ctx
+      type(slgpu_gas), intent(in) :: gas
+      integer(c_int), value :: flags
+      type(c_ptr), value :: cap64, cap32, sig, capgrey, capross, emiss
+    end function
+
+    integer(c_int) function slgpu_ddmc_prep(ctx, radtemp, lbol, opaclump, emitprob, emit, &
+         sthermal) bind(C, name="slgpu_ddmc_prep")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: radtemp(*)
+      real(c_double), value :: lbol
+      type(c_ptr), value :: opaclump, emitprob, emit, sthermal
+    end function
+
+    integer(c_int) function slgpu_sources(ctx, nstot, iteration, nvol, nnew) &
+         bind(C, name="slgpu_sources")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nstot
+      integer(c_int32_t), value :: iteration
+      type(c_ptr), value :: nvol
+      integer(c_int64_t), intent(out) :: nnew
+    end function
+
+    integer(c_int) function slgpu_transport(ctx, iteration, tally, jrad, totals, counters) &
+         bind(C, name="slgpu_transport")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: iteration
+      type(c_ptr), value :: tally, jrad, totals, counters
+    end function
+
+    integer(c_int) function slgpu_fluxtally(ctx, luminos, lumdev, lumnum) &
+         bind(C, name="slgpu_fluxtally")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: luminos, lumdev, lumnum
+    end function
+
+    integer(c_int) function slgpu_get_particles(ctx, n, stat, x, mu, wl, e, ig, nstep, err) &
+         bind(C, name="slgpu_get_particles")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: stat, x, mu, wl, e, ig, nstep, err
+    end function
+
+    integer(c_int) function slgpu_timers(ctx, ms, n, reset) bind(C, name="slgpu_timers")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms(*)
+      integer(c_int), value :: n, reset
+    end function
+
+    type(c_ptr) function slgpu_stream(ctx) bind(C, name="slgpu_stream")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+
+contains
+
+  !-- Fortran string of the last error
+  function slgpu_error_string() result(s)
+    character(len=:), allocatable :: s
+    character(kind=c_char), pointer :: p(:)
+    integer :: n
+    type(c_ptr) :: cp
+    cp = slgpu_last_error_message()
+    call c_f_pointer(cp, p, [4096])
+    n = 0
+    do while (n < 4096)
+      if (p(n+1) == c_null_char) exit
+      n = n + 1
+    enddo
+    allocate(character(len=n) :: s)
+    s = transfer(p(1:n), s)
+  end function
+
+end module slgpu_iface

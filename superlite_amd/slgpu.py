@@ -1,0 +1,245 @@
+"""ctypes binding of libslgpu.so (include/slgpu.h) and a host mirror of SuperLite's
+per-iteration call sequence (superlite.f90:148-219).
+
+The product path is the HIP library; this module only marshals host arrays.  Loading
+fails loudly if libslgpu.so has not been built (python superlite_amd/build.py).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libslgpu.so")
+
+SLGPU_FP64, SLGPU_MIXED = 0, 1
+NOBB, NOBF, NOFF, NOTHOMSON = 1, 2, 4, 8
+E_TRANSPORT = 4
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+
+EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_set_grid",
+           "slgpu_set_groups", "slgpu_set_lines", "slgpu_set_bf_tables", "slgpu_set_ff_table",
+           "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
+           "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_timers",
+           "slgpu_stream")
+
+
+class SlgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"slgpu error {code}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nx", _i), ("ng", _i), ("nep", _i), ("nepg", _i), ("tauddmc", _d),
+                ("taulump", _d), ("puretran", _i), ("errorfatal", _i), ("precision", _i),
+                ("seed", _u32), ("npartmax", _i64), ("max_steps", _i64)]
+
+
+class Gas(ctypes.Structure):
+    _fields_ = [("temp", _vp), ("natom", _vp), ("vol", _vp), ("mass", _vp), ("nelec", _vp),
+                ("natomfr", _vp), ("oc", _vp), ("ginv", _vp)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libslgpu.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built; run python superlite_amd/build.py")
+        L = ctypes.CDLL(LIB)
+        L.slgpu_last_error_message.restype = ctypes.c_char_p
+        L.slgpu_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(_vp)]
+        L.slgpu_destroy.argtypes = [_vp]
+        L.slgpu_set_grid.argtypes = [_vp] + [_vp] * 5
+        L.slgpu_set_groups.argtypes = [_vp, _vp]
+        L.slgpu_set_lines.argtypes = [_vp, _i64] + [_vp] * 5
+        L.slgpu_set_bf_tables.argtypes = [_vp, _vp, _vp]
+        L.slgpu_set_ff_table.argtypes = [_vp, _vp]
+        L.slgpu_set_ions.argtypes = [_vp, _i, _vp, _vp]
+        L.slgpu_opacity_build.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
+        L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
+        L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
+        L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp]
+        L.slgpu_fluxtally.argtypes = [_vp, _vp, _vp, _vp]
+        L.slgpu_get_particles.argtypes = [_vp, _i64] + [_vp] * 8
+        L.slgpu_timers.argtypes = [_vp, _vp, _i, _i]
+        L.slgpu_stream.restype = _vp
+        L.slgpu_stream.argtypes = [_vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Context:
+    """One slgpu context: device-resident mesh, groups, lines, opacities, particles."""
+
+    def __init__(self, nx, ng, nep, nepg, tauddmc=5.0, taulump=10.0, puretran=False,
+                 errorfatal=True, precision=SLGPU_FP64, seed=1, npartmax=1 << 20, max_steps=0):
+        self.L = lib()
+        self.nx, self.ng, self.nep, self.nepg = nx, ng, nep, nepg
+        cfg = Config(nx, ng, nep, nepg, tauddmc, taulump, int(puretran), int(errorfatal),
+                     precision, seed, npartmax, max_steps)
+        h = _vp()
+        self._check(self.L.slgpu_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.npart = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SlgpuError(rc, self.L.slgpu_last_error_message().decode())
+
+    def close(self):
+        if self.h:
+            self.L.slgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- static inputs ----------------------------------------------------
+    def set_grid(self, xarr, vxarr, divv, vol, rcell):
+        a = [_c(v, np.float64) for v in (xarr, vxarr, divv, vol, rcell)]
+        self._check(self.L.slgpu_set_grid(self.h, *[_p(x) for x in a]))
+
+    def set_groups(self, wl):
+        self._check(self.L.slgpu_set_groups(self.h, _p(_c(wl, np.float64))))
+
+    def set_lines(self, wl0, gxs, chilw, iz, ii):
+        a = [_c(wl0, np.float32), _c(gxs, np.float32), _c(chilw, np.float32),
+             _c(iz, np.int16), _c(ii, np.int16)]
+        self._check(self.L.slgpu_set_lines(self.h, len(a[0]), *[_p(x) for x in a]))
+
+    def set_tables(self, ph1, ph2, gff):
+        self._check(self.L.slgpu_set_bf_tables(self.h, _p(_c(ph1, np.float32)),
+                                               _p(_c(ph2, np.float32))))
+        self._check(self.L.slgpu_set_ff_table(self.h, _p(_c(gff, np.float64))))
+
+    def set_ions(self, ion_iz, ion_ii):
+        iz, ii = _c(ion_iz, np.int32), _c(ion_ii, np.int32)
+        self._check(self.L.slgpu_set_ions(self.h, len(iz), _p(iz), _p(ii)))
+
+    # -- per iteration ----------------------------------------------------
+    def opacity_build(self, temp, natom, vol, mass, nelec, natomfr, oc, ginv, flags=0,
+                      want=("cap64", "cap", "sig", "capgrey", "capross", "emiss")):
+        """physical_opacity + opacity_planckmean + opacity_rossmean.
+        natomfr (30, nc); oc, ginv (nc, nion).  Group-resolved outputs have shape (nc, ng)."""
+        nc, ng = self.nx, self.ng
+        keep = [_c(temp, np.float64), _c(natom, np.float64), _c(vol, np.float64),
+                _c(mass, np.float64), _c(nelec, np.float64), _c(np.asarray(natomfr).T, np.float64),
+                _c(oc, np.float64), _c(ginv, np.float64)]
+        gas = Gas(*[_p(x) for x in keep])
+        out = {}
+        if "cap64" in want: out["cap64"] = np.zeros((nc, ng))
+        if "cap" in want: out["cap"] = np.zeros((nc, ng), np.float32)
+        if "emiss" in want: out["emiss"] = np.zeros((nc, ng), np.float32)
+        for k in ("sig", "capgrey", "capross"):
+            if k in want: out[k] = np.zeros(nc)
+        self._check(self.L.slgpu_opacity_build(self.h, ctypes.byref(gas), flags,
+                                               _p(out.get("cap64")), _p(out.get("cap")),
+                                               _p(out.get("sig")), _p(out.get("capgrey")),
+                                               _p(out.get("capross")), _p(out.get("emiss"))))
+        return out
+
+    def ddmc_prep(self, radtemp, lbol, want=True):
+        nc = self.nx
+        out = dict(opaclump=np.zeros((nc, 11)), emitprob=np.zeros((nc, self.nep)),
+                   emit=np.zeros(nc)) if want else {}
+        st = _d()
+        self._check(self.L.slgpu_ddmc_prep(self.h, _p(_c(radtemp, np.float64)), float(lbol),
+                                           _p(out.get("opaclump")), _p(out.get("emitprob")),
+                                           _p(out.get("emit")), ctypes.byref(st)))
+        out["sthermal"] = st.value
+        return out
+
+    def sources(self, nstot, iteration):
+        nvol = np.zeros(self.nx, np.int32)
+        nnew = _i64()
+        self._check(self.L.slgpu_sources(self.h, int(nstot), iteration, _p(nvol), ctypes.byref(nnew)))
+        self.npart = nnew.value
+        return nvol, nnew.value
+
+    def transport(self, iteration, want=True, check=True):
+        nc, ng = self.nx, self.ng
+        tally = np.zeros(nc) if want else None
+        jrad = np.zeros((nc, ng)) if want else None
+        tot = np.zeros(6)
+        ct = np.zeros(10, np.int64)
+        rc = self.L.slgpu_transport(self.h, iteration, _p(tally), _p(jrad), _p(tot), _p(ct))
+        out = dict(tally=tally, jrad=jrad, eout=tot[0], evelo=tot[1], sflux=tot[2], estart=tot[3],
+                   edead=tot[4], esrc=tot[5], nactive=int(ct[0]), nstepimc=int(ct[1]),
+                   nstepddmc=int(ct[2]), nstepmax=int(ct[3]), nmethswap=int(ct[4]),
+                   nflux=int(ct[5]), ndead=int(ct[6]), nerr=int(ct[7]), nwarn=int(ct[8]), rc=rc)
+        if check and rc != 0:
+            self._check(rc)
+        return out
+
+    def fluxtally(self):
+        ng = self.ng
+        lum, dev, num = np.zeros(ng), np.zeros(ng), np.zeros(ng, np.int32)
+        self._check(self.L.slgpu_fluxtally(self.h, _p(lum), _p(dev), _p(num)))
+        return dict(luminos=lum, lumdev=dev, lumnum=num)
+
+    def particles(self, n=None):
+        n = self.npart if n is None else n
+        pp = dict(stat=np.zeros(n, np.uint8), x=np.zeros(n), mu=np.zeros(n), wl=np.zeros(n),
+                  e=np.zeros(n), ig=np.zeros(n, np.int32), nstep=np.zeros(n, np.int64),
+                  err=np.zeros(n, np.int32))
+        self._check(self.L.slgpu_get_particles(self.h, n, *[_p(pp[k]) for k in
+                                                            ("stat", "x", "mu", "wl", "e", "ig", "nstep", "err")]))
+        return pp
+
+    def timers(self, reset=False):
+        ms = np.zeros(8)
+        self._check(self.L.slgpu_timers(self.h, _p(ms), 8, int(reset)))
+        return dict(opacity=ms[0], prep=ms[1], sources=ms[2], transport=ms[3], fluxtally=ms[4],
+                    opacity_kernel=ms[5], transport_kernel=ms[6])
+
+
+def setup_workload(w, st, wl, lines, tables, precision=SLGPU_FP64, seed=1, npartmax=None,
+                   puretran=False, errorfatal=True, max_steps=0):
+    """Create a context holding one workload's static inputs (superlite.f90:87-131)."""
+    npm = npartmax if npartmax is not None else (1 << w.n2s)
+    ctx = Context(w.nx, w.ng, int(st["nep"]), int(st["nepg"]), w.tauddmc, w.taulump,
+                  puretran=puretran, errorfatal=errorfatal, precision=precision, seed=seed,
+                  npartmax=npm, max_steps=max_steps)
+    ctx.set_grid(st["xarr"], st["vxarr"], st["divv"], st["vol"], st["rcell"])
+    ctx.set_groups(wl)
+    ctx.set_ions(st["ion_iz"], st["ion_ii"])
+    ctx.set_tables(*tables)
+    ctx.set_lines(*lines)
+    return ctx
+
+
+def iteration(ctx, st, nstot, it, lbol, want=True, flags=0):
+    """One SuperLite iteration on the device, in the order of superlite.f90:157-219:
+    gas_update (opacity + means), sourceenergy/leakage/emission_probability, sources,
+    particle_advance, fluxtally.  Returns the tallies."""
+    op = ctx.opacity_build(st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"],
+                           st["natomfr"], st["oc"], st["ginv"], flags=flags,
+                           want=("cap", "sig", "capgrey", "capross") if want else ())
+    prep = ctx.ddmc_prep(st["temp"], lbol, want=want)
+    nvol, nnew = ctx.sources(nstot, it)
+    tr = ctx.transport(it, want=want)
+    fl = ctx.fluxtally() if want else ctx.fluxtally()
+    return dict(opacity=op, prep=prep, nvol=nvol, nnew=nnew, transport=tr, flux=fl)

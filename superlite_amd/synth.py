@@ -1,0 +1,175 @@
+"""Synthetic homologous-ejecta workloads for the SuperLite hot path.
+
+Each workload is what SuperLite's host would hold in its module state before the
+per-iteration loop (superlite.f90:148): the 1D-spherical grid (gridmod grd_xarr,
+grd_vxarr), the group grid (groupmod grp_wl), per-shell gas state (gasmod gas_mass,
+gas_temp, mass fractions) and a wavelength-sorted line list with the field layout of
+bbxsmod.f:27-34 (wl0 [A] f32, gxs f32, chilw [1/cm] f32, iz i16, ii i16).
+
+The ionisation balance (gas_natom, gas_natomfr, gas_nelec, ion_grndlev oc/ginv) is an
+input of the opacity build, produced by the host's LTE EOS (GAS/eos_update.f:23-45).
+For the named workloads it is loaded from the committed per-workload files under
+tests/golden/ (written by tests/golden/make_golden.py with SuperLite's own EOS).
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+
+import numpy as np
+
+# physconstmod.f:11-51
+PC_C = 2.997924562e10
+PC_PI = 3.1415926535897932385
+PC_E = 4.80325e-10
+PC_ME = 9.1093826e-28
+PC_H = 6.6260693e-27
+PC_MSUN = 1.989e33
+PC_EV = 1.60217653e-12
+PC_DAY = 24.0 * 60.0 ** 2
+
+T_SNAP = 20.0 * PC_DAY          # 20 days, homologous snapshot time
+V_BREAK = 5000e5                # 5000 km/s, density break
+M_TOT = 1.4 * PC_MSUN
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "tests", "golden")
+
+
+@dataclasses.dataclass
+class Workload:
+    """One synthetic configuration (sizes and seeds)."""
+    name: str
+    nx: int
+    vmax: float            # cm/s
+    ng: int
+    wlmin: float           # cm
+    wlmax: float           # cm
+    nline: int
+    n2s: int               # log2 particles per iteration
+    niter: int
+    elements: tuple        # atomic numbers present
+    massfr: tuple | None   # uniform mass fractions, or None for seeded Dirichlet per shell
+    csm: tuple | None = None   # (v0, v1, rho factor, T peak) of a dense shell
+    lbol: float = 1e43
+    tauddmc: float = 5.0
+    taulump: float = 10.0
+    seed_lines: int = 1001
+    seed_ejecta: int = 1002
+
+
+WORKLOADS = {
+    # 50 shells to 20000 km/s, H/He/O/Fe 0/0.2/0.6/0.2, 1000 groups 1000-32000 A, 1e5 lines, 2^20 particles
+    "c1": Workload("c1", 50, 2.0e9, 1000, 1000e-8, 32000e-8, 100_000, 20, 10,
+                   (1, 2, 8, 26), (0.0, 0.2, 0.6, 0.2)),
+    # 400 shells to 30000 km/s, 8 elements Dirichlet, 10^4 groups 500-50000 A, 2e6 lines, 2^24 particles
+    "c2": Workload("c2", 400, 3.0e9, 10_000, 500e-8, 50000e-8, 2_000_000, 24, 20,
+                   (1, 2, 6, 8, 14, 16, 20, 26), None),
+    # 1000 shells, dense CSM shell at 8000-9000 km/s (rho x1e3, T to 5e4 K), 4e6 lines, 2^26 particles
+    "c5": Workload("c5", 1000, 3.0e9, 10_000, 500e-8, 50000e-8, 4_000_000, 26, 20,
+                   (1, 2, 6, 8, 14, 16, 20, 26), None, csm=(8000e5, 9000e5, 1e3, 5e4)),
+}
+
+
+def group_grid(w: Workload) -> np.ndarray:
+    """Log-spaced group edges, grp_wl(ig) = wlmin*(wlmax/wlmin)**((ig-1)/ng) (groupmod.f:46-47)."""
+    ig = np.arange(w.ng + 1, dtype=np.float64)
+    return w.wlmin * (w.wlmax / w.wlmin) ** (ig / w.ng)
+
+
+def ejecta(w: Workload):
+    """Grid edges, shell masses, temperatures and mass fractions (30, nx).
+
+    Density rho ~ (v/vb)^-1 inside vb and (v/vb)^-7 outside, continuous at vb; shell
+    masses are the exact integrals 4 pi t^3 int rho v^2 dv normalised to 1.4 Msun.
+    T at the shell centre falls linearly from 1.2e4 K (v=0) to 4e3 K (vmax).
+    """
+    v = np.linspace(0.0, w.vmax, w.nx + 1)
+    x = v * T_SNAP
+
+    def prim(u):  # antiderivative of (rho/A) v^2 in units of vb^3
+        u = np.asarray(u, dtype=np.float64)
+        inner = 0.5 * np.minimum(u, 1.0) ** 2
+        outer = np.where(u > 1.0, 0.25 * (1.0 - np.maximum(u, 1.0) ** -4), 0.0)
+        return inner + outer
+
+    u = v / V_BREAK
+    m = prim(u[1:]) - prim(u[:-1])
+    vc = 0.5 * (v[1:] + v[:-1])
+    temp = 1.2e4 + (4.0e3 - 1.2e4) * vc / w.vmax
+    m = m * (M_TOT / m.sum())
+    if w.csm is not None:
+        # dense shell on top of the 1.4 Msun ejecta: rho x fac, T raised towards tpk
+        v0, v1, fac, tpk = w.csm
+        inshell = (vc >= v0) & (vc <= v1)
+        m = np.where(inshell, m * fac, m)
+        vm = 0.5 * (v0 + v1)
+        bump = np.clip(1.0 - np.abs(vc - vm) / (0.5 * (v1 - v0)), 0.0, 1.0)
+        temp = np.where(inshell, temp + (tpk - temp) * bump, temp)
+
+    massfr = np.zeros((w.nx, 30))
+    if w.massfr is not None:
+        for z, f in zip(w.elements, w.massfr):
+            massfr[:, z - 1] = f
+    else:
+        rng = np.random.default_rng(w.seed_ejecta)
+        d = rng.dirichlet(np.ones(len(w.elements)), size=w.nx)
+        for k, z in enumerate(w.elements):
+            massfr[:, z - 1] = d[:, k]
+    return x, v, m, temp, massfr.T.copy()
+
+
+def ion_list(w: Workload, ni: np.ndarray):
+    """(iz, ii) for every line/bf ion of the present elements, ii <= min(iz, ni(iz)-1)."""
+    izs, iis = [], []
+    for z in w.elements:
+        for ii in range(1, min(z, int(ni[z - 1]) - 1) + 1):
+            izs.append(z)
+            iis.append(ii)
+    return np.array(izs, dtype=np.int32), np.array(iis, dtype=np.int32)
+
+
+def line_list(w: Workload, ion_iz: np.ndarray, ion_ii: np.ndarray):
+    """Seeded synthetic line list, lambda-sorted, quantised like data.atom records.
+
+    lambda log-uniform in [wlmin, wlmax]; log gf ~ N(-2, 1.5) kept to 3 decimals (f7.3,
+    bbxsmod.f:88); lower-level energy U(0, 10) eV kept to 3 decimals in cm^-1 (f11.3,
+    bbxsmod.f:82); gxs = fconst * 10**loggf in single precision with
+    fconst = sngl(pi e^2/(m_e c)) (read_bbxs_data.f:15,60-66).
+    """
+    rng = np.random.default_rng(w.seed_lines)
+    n = w.nline
+    lnwl = rng.uniform(np.log(w.wlmin / 1e-8), np.log(w.wlmax / 1e-8), n)
+    wl0 = np.exp(lnwl).astype(np.float32)
+    loggf = np.round(rng.normal(-2.0, 1.5, n), 3).astype(np.float32)
+    elow_ev = rng.uniform(0.0, 10.0, n)
+    chilw = np.round(elow_ev * PC_EV / (PC_H * PC_C), 3).astype(np.float32)
+    fconst = np.float32(PC_PI * PC_E ** 2 / (PC_ME * PC_C))
+    gxs = (fconst * np.power(np.float32(10.0), loggf)).astype(np.float32)
+    pick = rng.integers(0, len(ion_iz), n)
+    iz = ion_iz[pick].astype(np.int16)
+    ii = ion_ii[pick].astype(np.int16)
+    order = np.argsort(wl0, kind="stable")
+    return (wl0[order].copy(), gxs[order].copy(), chilw[order].copy(),
+            iz[order].copy(), ii[order].copy())
+
+
+def load_tables():
+    """Verner bf fit tables (6,7,30,30)/(7,30,30) f32 and Sutherland g_ff (81,41) f64,
+    Fortran order, as parsed by SuperLite's own readers (bfxsmod.f:25-61, ffxsmod.f:28-54)."""
+    d = np.load(os.path.join(GOLDEN_DIR, "tables.npz"))
+    return d["bf_ph1"], d["bf_ph2"], d["ff_gff"]
+
+
+def load_state(name: str):
+    """Host gas state of a named workload (ionisation balance from SuperLite's EOS)."""
+    return dict(np.load(os.path.join(GOLDEN_DIR, f"state_{name}.npz")))
+
+
+def build(name: str):
+    """Everything the host holds for one named workload, as numpy arrays."""
+    w = WORKLOADS[name]
+    st = load_state(name)
+    wl = group_grid(w)
+    lines = line_list(w, st["ion_iz"], st["ion_ii"])
+    return w, st, wl, lines

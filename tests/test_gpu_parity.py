@@ -1,0 +1,279 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle and against
+SuperLite's own outputs in tests/golden/.  Marked gpu; run on an MI355X."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from superlite_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def S():
+    from superlite_amd import slgpu
+    return slgpu
+
+
+def make_ctx(S, name, tables, **kw):
+    w, st, wl, lines = synth.build(name)
+    ctx = S.setup_workload(w, st, wl, lines, tables, **kw)
+    return ctx, w, st, wl, lines
+
+
+def gpu_opacity(ctx, st, flags=0):
+    return ctx.opacity_build(st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"],
+                             st["natomfr"], st["oc"], st["ginv"], flags=flags)
+
+
+def test_library_exports(S):
+    L = S.lib()
+    for name in S.EXPORTS:
+        assert hasattr(L, name)
+
+
+def test_opacity_c1(S, tables, golden):
+    say("case: opacity c1")
+    ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1024)
+    g = gpu_opacity(ctx, st)
+    o = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"],
+                   st["ion_iz"], st["ion_ii"], st["oc"], st["ginv"], lines, tables)
+    rel = np.abs(g["cap64"] - o["cap64"]) / np.abs(o["cap64"])
+    say("c1 cap64 max rel vs oracle", rel.max())
+    assert rel.max() < 1e-12
+    ref = golden("ref_c1_opacity.npz")
+    assert ulp_diff(g["cap"], ref["cap"]).max() <= 1
+    assert ulp_diff(g["emiss"], ref["emiss"]).max() <= 1
+    np.testing.assert_allclose(g["sig"], ref["sig"], rtol=1e-14)
+    np.testing.assert_allclose(g["capgrey"], ref["capgrey"], rtol=1e-12)
+    np.testing.assert_allclose(g["capross"], ref["capross"], rtol=1e-12)
+    ctx.close()
+
+
+@pytest.mark.parametrize("flags", [0, 6, 5, 3, 8])
+def test_opacity_edge(S, tables, golden, flags):
+    say("case: opacity edge flags", flags)
+    e = golden("ref_edge_opacity.npz")
+    ctx = S.Context(3, 5, 2, 2, npartmax=16)
+    ctx.set_grid(e["xarr"], e["vxarr"], e["divv"], e["vol"], 0.5 * (e["xarr"][1:] + e["xarr"][:-1]))
+    ctx.set_groups(e["wl"])
+    ctx.set_ions(e["ion_iz"], e["ion_ii"])
+    ctx.set_tables(*tables)
+    ctx.set_lines(e["wl0"], e["gxs"], e["chilw"], e["liz"], e["lii"])
+    g = ctx.opacity_build(e["temp"], e["natom"], e["vol"], e["mass"], e["nelec"], e["natomfr"],
+                          e["oc"], e["ginv"], flags=flags)
+    o = po.opacity(e["wl"], e["temp"], e["natom"], e["vol"], e["mass"], e["nelec"], e["natomfr"],
+                   e["ion_iz"], e["ion_ii"], e["oc"], e["ginv"],
+                   (e["wl0"], e["gxs"], e["chilw"], e["liz"], e["lii"]), tables, flags=flags)
+    nz = o["cap64"] != 0
+    assert np.array_equal(g["cap64"] != 0, nz)
+    np.testing.assert_allclose(g["cap64"][nz], o["cap64"][nz], rtol=1e-12)
+    if f"cap_{flags}" in e:
+        assert ulp_diff(g["cap"], e[f"cap_{flags}"]).max() <= 1
+    np.testing.assert_allclose(g["sig"], o["sig"], rtol=1e-14)
+    ctx.close()
+
+
+def test_opacity_c2_full_size(S, tables):
+    """production size: 400 shells x 10^4 groups x 2e6 lines; oracle on 16 of the shells"""
+    say("case: opacity c2")
+    ctx, w, st, wl, lines = make_ctx(S, "c2", tables, npartmax=1024)
+    g = gpu_opacity(ctx, st)
+    cells = np.arange(0, w.nx, 25)
+    sub = {k: st[k][cells] for k in ("temp", "natom", "vol", "mass", "nelec", "oc", "ginv")}
+    o = po.opacity(wl, sub["temp"], sub["natom"], sub["vol"], sub["mass"], sub["nelec"],
+                   st["natomfr"][:, cells], st["ion_iz"], st["ion_ii"], sub["oc"], sub["ginv"],
+                   lines, tables)
+    rel = np.abs(g["cap64"][cells] - o["cap64"]) / np.abs(o["cap64"])
+    say("c2 cap64 max rel vs oracle", rel.max())
+    assert rel.max() < 1e-12
+    assert ulp_diff(g["cap"][cells], o["cap"]).max() <= 1
+    np.testing.assert_allclose(g["capgrey"][cells], o["capgrey"], rtol=1e-12)
+    np.testing.assert_allclose(g["capross"][cells], o["capross"], rtol=1e-12)
+    ctx.close()
+
+
+def oracle_model(st, w, wl, g, prep=None, nvol=None, **kw):
+    m = po.Model(st["xarr"], st["vxarr"], st["divv"], st["vol"], wl, int(st["nep"]),
+                 int(st["nepg"]), g["cap"], g["sig"], g["capgrey"], 1.0 / st["temp"], st["temp"],
+                 tauddmc=w.tauddmc, taulump=w.taulump, **kw)
+    if prep is not None:
+        m.set(opaclump=prep["opaclump"], emitprob=prep["emitprob"], emit=prep["emit"])
+    if nvol is not None:
+        m.set(nvol=nvol)
+    return m
+
+
+def test_ddmc_prep_c1(S, tables):
+    say("case: ddmc prep c1")
+    ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1024)
+    g = gpu_opacity(ctx, st)
+    p = ctx.ddmc_prep(st["temp"], w.lbol)
+    m = oracle_model(st, w, wl, g)
+    ol = m.leakage()
+    for k in (0, 1, 6, 7, 9, 10):
+        np.testing.assert_allclose(p["opaclump"][:, k], ol[:, k], rtol=1e-12, atol=0,
+                                   err_msg=f"opaclump({k + 1})")
+    np.testing.assert_allclose(p["emitprob"], m.emission_probability(), rtol=1e-12)
+    emit, sth = po.sourceenergy(st["rcell"], st["temp"], w.lbol)
+    np.testing.assert_allclose(p["emit"], emit, rtol=1e-15)
+    assert p["sthermal"] == pytest.approx(w.lbol, rel=1e-12)
+    ctx.close()
+
+
+def test_sources_c1_per_particle(S, tables):
+    say("case: sources c1")
+    ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 14, seed=5)
+    g = gpu_opacity(ctx, st)
+    p = ctx.ddmc_prep(st["temp"], w.lbol)
+    nvol, nnew = ctx.sources(1 << 14, 3)
+    assert nnew == 1 << 14
+    assert np.array_equal(nvol, po.sourcenumbers(p["emit"], 1 << 14))
+    m = oracle_model(st, w, wl, g, prep=p, nvol=nvol)
+    o = m.sources(5, 3)
+    # the GPU keeps the sampled state in its particle store; read it back before transport
+    pp = ctx.particles(nnew)
+    np.testing.assert_allclose(pp["x"], o["x"], rtol=1e-13)
+    np.testing.assert_allclose(pp["mu"], o["mu"], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(pp["e"], o["e"], rtol=1e-14)
+    same_wl = np.isclose(pp["wl"], o["wl"], rtol=1e-13)
+    assert same_wl.mean() > 0.9999, same_wl.mean()
+    ctx.close()
+
+
+def run_both(S, tables, n2s, seed, it, scale=1.0, max_steps=0):
+    """GPU iteration, and the oracle iteration on the GPU's own per-iteration state."""
+    ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << n2s, seed=seed,
+                                     max_steps=max_steps)
+    st = dict(st)
+    st["oc"] = st["oc"] * scale
+    st["nelec"] = st["nelec"] * scale
+    flags = 0 if scale == 1.0 else S.NOFF
+    g = ctx.opacity_build(st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"],
+                          st["natomfr"], st["oc"], st["ginv"], flags=flags)
+    p = ctx.ddmc_prep(st["temp"], w.lbol)
+    nvol, nnew = ctx.sources(1 << n2s, it)
+    tr = ctx.transport(it)
+    fl = ctx.fluxtally()
+    pp = ctx.particles(nnew)
+    m = oracle_model(st, w, wl, g, prep=p, nvol=nvol, max_steps=max_steps)
+    o = m.iteration(seed, it, per_particle=True)
+    ctx.close()
+    return tr, fl, pp, o
+
+
+@pytest.mark.parametrize("scale", [1e-5, 1.0])
+def test_transport_c1_per_particle(S, tables, scale):
+    say("case: transport c1 per particle, scale", scale)
+    tr, fl, pp, o = run_both(S, tables, 12, 7, 2, scale=scale)
+    op = o["particles"]
+    assert tr["nerr"] == 0 and o["nerr"] == 0
+    same = (pp["stat"] == op["stat"]) & (pp["ig"] == op["ig"]) & \
+        np.isclose(pp["e"], op["e"], rtol=1e-9) & np.isclose(pp["wl"], op["wl"], rtol=1e-9)
+    say("identical trajectories", same.mean(), "steps gpu", tr["nstepimc"], tr["nstepddmc"],
+        "oracle", o["nstepimc"], o["nstepddmc"])
+    assert same.mean() > 0.995
+    # energy bookkeeping: every particle's start energy ends as escaped, adiabatic or dead
+    assert tr["estart"] == pytest.approx(tr["eout"] + tr["evelo"] + tr["edead"], rel=1e-12)
+    assert tr["esrc"] == pytest.approx(o["esrc"], rel=1e-12)
+    assert tr["nactive"] == o["nactive"]
+    # flux tally of the device equals the binning of the device's own final states
+    flux = pp["stat"] == 1
+    ig = np.clip(np.searchsorted(synth.group_grid(synth.WORKLOADS["c1"]), pp["wl"][flux],
+                                 side="right"), 1, 1000)
+    lum = np.bincount(ig - 1, weights=pp["e"][flux], minlength=1000)
+    np.testing.assert_allclose(fl["luminos"], lum, rtol=1e-12, atol=1e-300)
+    assert fl["lumnum"].sum() == flux.sum() == tr["nflux"]
+    np.testing.assert_allclose(fl["luminos"].sum(), tr["eout"], rtol=1e-12)
+    # tallies agree with the oracle's to the trajectory-divergence level
+    np.testing.assert_allclose(tr["tally"], o["tally"], rtol=2e-2)
+    assert abs(tr["eout"] - o["eout"]) / o["eout"] < 2e-2
+
+
+def coarse(a, nb=20):
+    a = np.asarray(a)
+    n = a.shape[-1] - a.shape[-1] % nb
+    return a[..., :n].reshape(a.shape[:-1] + (nb, -1)).sum(-1)
+
+
+@pytest.mark.parametrize("case", ["c1mixed", "c1"])
+def test_transport_statistics_vs_reference(S, tables, golden, case):
+    """MC tallies within 3 sigma of SuperLite's own Monte Carlo (golden per-iteration rows)."""
+    say("case: transport statistics", case)
+    g = golden(f"ref_{case}_mc.npz")
+    scale = float(g["scale"])
+    n2s = int(g["n2s"])
+    w, st, wl, lines = synth.build("c1")
+    st = dict(st)
+    st["oc"] = st["oc"] * scale
+    st["nelec"] = st["nelec"] * scale
+    ctx = S.setup_workload(w, st, wl, lines, tables, npartmax=1 << n2s, seed=21)
+    rows = []
+    for it in range(12):
+        r = S.iteration(ctx, st, 1 << n2s, it + 1, w.lbol, flags=int(g["flags"]))
+        rows.append(dict(luminos=r["flux"]["luminos"], tally=r["transport"]["tally"],
+                         eout=r["transport"]["eout"], evelo=r["transport"]["evelo"]))
+        t = r["transport"]
+        assert t["estart"] == pytest.approx(t["eout"] + t["evelo"] + t["edead"], rel=1e-12)
+    ctx.close()
+
+    def z(ref, ours):
+        a, b = np.asarray(ref, float), np.asarray(ours, float)
+        ma, sa = a.mean(0), a.std(0, ddof=1) / np.sqrt(len(a))
+        mb, sb = b.mean(0), b.std(0, ddof=1) / np.sqrt(len(b))
+        s = np.sqrt(sa ** 2 + sb ** 2)
+        ok = s > 0
+        return (ma[ok] - mb[ok]) / s[ok]
+    for key, f in (("luminos", coarse), ("tally", lambda a: coarse(a, 10))):
+        zz = z(f(g[key]), [f(r[key]) for r in rows])
+        say(case, key, "chi2/dof", np.mean(zz ** 2), "max|z|", np.max(np.abs(zz)))
+        assert np.mean(zz ** 2) < 2.5
+        assert np.max(np.abs(zz)) < 4.5
+    for key in ("eout", "evelo"):
+        zz = z(g[key][:, None], [[r[key]] for r in rows])
+        assert abs(zz[0]) < 4.0, (key, zz)
+
+
+def test_transport_error_policy(S, tables):
+    """step cap reached: fatal with errorfatal (particle_advance.f90:262-263), else 'dead'"""
+    say("case: error policy")
+    w, st, wl, lines = synth.build("c1")
+    for fatal in (True, False):
+        ctx = S.setup_workload(w, st, wl, lines, tables, npartmax=256, seed=3, max_steps=3,
+                               errorfatal=fatal)
+        gpu_opacity(ctx, st)
+        ctx.ddmc_prep(st["temp"], w.lbol)
+        ctx.sources(256, 1)
+        tr = ctx.transport(1, check=False)
+        assert tr["nerr"] > 0
+        assert tr["ndead"] == tr["nerr"]
+        assert (tr["rc"] == S.E_TRANSPORT) == fatal
+        assert tr["estart"] == pytest.approx(tr["eout"] + tr["evelo"] + tr["edead"], rel=1e-12)
+        pp = ctx.particles()
+        assert set(np.unique(pp["err"][pp["stat"] == 2])) <= {200}
+        ctx.close()
+
+
+def test_call_order_errors(S, tables):
+    w, st, wl, lines = synth.build("c1")
+    ctx = S.Context(w.nx, w.ng, int(st["nep"]), int(st["nepg"]), npartmax=64)
+    with pytest.raises(S.SlgpuError) as ei:
+        ctx.sources(64, 1)
+    assert ei.value.code == 2
+    with pytest.raises(S.SlgpuError):
+        ctx.set_lines(lines[0][::-1].copy(), *lines[1:])   # unsorted line list
+    ctx.close()

@@ -1,0 +1,234 @@
+"""Pins of the CPU oracle (oracle/oracle.c) against SuperLite's own outputs (golden files
+written by tests/golden/make_golden.py from the reference compiled with flang), published
+known-answer vectors, closed forms and long-double brute force.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from superlite_amd import synth
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+# --- Philox-4x32-10 known answers (Random123 kat_vectors, Salmon et al. SC'11) -------
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_kat(ctr, key, expect):
+    import ctypes
+
+    class U4(ctypes.Structure):
+        _fields_ = [("v", ctypes.c_uint32 * 4)]
+    L = po.lib()
+    f = L.or_philox_block
+    f.restype = U4
+    f.argtypes = [U4, ctypes.c_uint32, ctypes.c_uint32]
+    c = U4((ctypes.c_uint32 * 4)(*ctr))
+    out = f(c, key[0], key[1])
+    assert tuple(out.v) == expect
+
+
+# --- binsrch (MISC/binsrch.f90) -----------------------------------------------------
+def test_binsrch_golden(golden):
+    g = golden("ref_c1_opacity.npz")
+    for k, wr in enumerate((0, 1)):
+        got = [po.binsrch(x, g["bs_arr"], wr) for x in g["bs_x"]]
+        assert got == list(g["bs_w"][k])
+        got2 = [po.binsrch(x, [1.0, 2.0], wr) for x in (0.5, 1.0, 1.5, 2.0, 2.5)]
+        assert got2 == list(g["bs2"][k])
+
+
+def test_binsrch_bruteforce():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        n = int(rng.integers(3, 40))
+        arr = np.sort(rng.integers(0, 20, n)).astype(float)
+        for x in rng.uniform(-2, 22, 20):
+            exp = 0 if x < arr[0] else (n if x > arr[-1] else
+                                       min(int(np.searchsorted(arr, x, side="right")), n - 1))
+            assert po.binsrch(x, arr, 1) == exp
+            assert po.binsrch(x, arr, 0) == min(max(exp, 1), n - 1)
+
+
+# --- Planck group integrals (groupmod.f) -------------------------------------------
+def test_specintv_normalisation():
+    wl = np.geomspace(1e-7, 1e1, 20001)
+    s = po.specintv(1.0 / 1e4, 1.0 / wl)
+    assert abs(s.sum() - 1.0) < 1e-6            # int x^3/(e^x-1) dx = pi^4/15
+    t = po.spectintv(1.0 / 1e4, 1.0 / wl)
+    assert abs(t.sum() - 1.0) < 1e-6            # int x^4 e^x/(e^x-1)^2 dx = 4 pi^4/15
+    # x >= 300 is capped, not overflowed
+    assert np.all(np.isfinite(po.specintv(1.0 / 10.0, 1.0 / wl)))
+
+
+def test_specint0_matches_specintv_element():
+    wl = synth.group_grid(synth.WORKLOADS["c1"])
+    wlinv = 1.0 / wl
+    s = po.specintv(1 / 7000.0, wlinv)
+    for ig in (1, 17, 500, 1000):
+        assert po.specint0(1 / 7000.0, wlinv, ig) == pytest.approx(s[ig - 1], rel=1e-13)
+
+
+# --- bf: Verner fit (bfxsmod.f) -----------------------------------------------------
+def test_bfxs_hydrogen_threshold(tables):
+    ph1, ph2, _ = tables
+    # H I ground-state photoionisation cross section at threshold is 6.3 Mb
+    assert po.bfxs(1, 1, 13.61, ph1, ph2) == pytest.approx(6.30, rel=0.02)
+    assert po.bfxs(1, 1, 13.5, ph1, ph2) == 0.0       # below threshold
+    # hydrogenic E^-3 falloff far above threshold (log slope ~ -3 near 1 keV)
+    a, b = po.bfxs(1, 1, 1000.0, ph1, ph2), po.bfxs(1, 1, 2000.0, ph1, ph2)
+    assert -3.6 < np.log2(b / a) < -2.9
+
+
+# --- opacity vs SuperLite's own output -------------------------------------------
+def test_opacity_c1_vs_reference(c1, tables, golden):
+    w, st, wl, lines = c1
+    g = golden("ref_c1_opacity.npz")
+    o = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"],
+                   st["ion_iz"], st["ion_ii"], st["oc"], st["ginv"], lines, tables)
+    assert ulp_diff(o["cap"], g["cap"]).max() <= 1
+    assert ulp_diff(o["emiss"], g["emiss"]).max() <= 1
+    np.testing.assert_allclose(o["sig"], g["sig"], rtol=1e-14)
+    np.testing.assert_allclose(o["capgrey"], g["capgrey"], rtol=1e-13)
+    np.testing.assert_allclose(o["capross"], g["capross"], rtol=1e-13)
+    # FP64 cap rounds to the reference's FP32 cap
+    assert np.max(np.abs(o["cap64"] - g["cap"]) / g["cap"]) < 6.0e-8
+
+
+@pytest.mark.parametrize("flags", [0, 6, 5, 3])
+def test_opacity_edge_cases_vs_reference(golden, tables, flags):
+    e = golden("ref_edge_opacity.npz")
+    o = po.opacity(e["wl"], e["temp"], e["natom"], e["vol"], e["mass"], e["nelec"], e["natomfr"],
+                   e["ion_iz"], e["ion_ii"], e["oc"], e["ginv"],
+                   (e["wl0"], e["gxs"], e["chilw"], e["liz"], e["lii"]), tables, flags=flags)
+    assert ulp_diff(o["cap"], e[f"cap_{flags}"]).max() <= 1
+    np.testing.assert_allclose(o["sig"], e[f"sig_{flags}"], rtol=1e-14)
+    ok = np.isfinite(e[f"capgrey_{flags}"])
+    np.testing.assert_allclose(o["capgrey"][ok], e[f"capgrey_{flags}"][ok], rtol=1e-13)
+
+
+def test_bb_longdouble_bruteforce(golden, tables):
+    """bound-bound sum restated independently in extended precision on the edge case."""
+    e = golden("ref_edge_opacity.npz")
+    o = po.opacity(e["wl"], e["temp"], e["natom"], e["vol"], e["mass"], e["nelec"], e["natomfr"],
+                   e["ion_iz"], e["ion_ii"], e["oc"], e["ginv"],
+                   (e["wl0"], e["gxs"], e["chilw"], e["liz"], e["lii"]), tables, flags=6)
+    L = np.longdouble
+    h, c, kb = L("6.6260693e-27"), L("2.997924562e10"), L("1.3806505e-16")
+    wl = e["wl"].astype(L)
+    ref = np.zeros((3, 5), L)
+    ions = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(e["ion_iz"], e["ion_ii"]))}
+    for l in range(len(e["wl0"])):
+        # group membership uses the FP64 product wl0*pc_ang (physical_opacity.f:114), which
+        # decides the lines placed exactly on a group edge
+        w0 = float(e["wl0"][l]) * 1e-8
+        g = int(np.searchsorted(e["wl"], w0, side="right"))     # wl[g-1] <= w0 < wl[g]
+        if g < 1 or g > 5:
+            continue
+        k = ions[(int(e["liz"][l]), int(e["lii"][l]))]
+        wc = (wl[g - 1] + wl[g]) / 2
+        for i in range(3):
+            if e["mass"][i] <= 0:
+                continue
+            ocg = L(e["oc"][i, k]) * L(e["ginv"][i, k])
+            if ocg <= 0:
+                continue
+            hckt = h * c / (kb * L(e["temp"][i]))
+            phi = wc * wc / ((wl[g] - wl[g - 1]) * c)
+            ref[i, g - 1] += (phi * L(float(e["gxs"][l])) * ocg *
+                              np.exp(-L(float(e["chilw"][l])) * hckt) * (1 - np.exp(-hckt / wc)))
+    m = ref > 0
+    np.testing.assert_allclose(o["cap64"][m], ref[m].astype(float), rtol=1e-13)
+    assert np.all(o["cap64"][~m] == 0)
+
+
+def test_thomson_constant(c1, tables):
+    w, st, wl, lines = c1
+    o = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"],
+                   st["ion_iz"], st["ion_ii"], st["oc"], st["ginv"], lines, tables, flags=7)
+    sigma_t = o["sig"] / (st["nelec"] * st["natom"] / st["vol"])
+    np.testing.assert_allclose(sigma_t, 6.6524e-25, rtol=2e-4)   # Thomson cross section
+
+
+# --- DDMC prep vs SuperLite ------------------------------------------------------
+def _model(st, w, wl, g, **kw):
+    return po.Model(st["xarr"], st["vxarr"], st["divv"], st["vol"], wl, int(st["nep"]),
+                    int(st["nepg"]), g["cap"], g["sig"], g["capgrey"], 1.0 / st["temp"],
+                    st["temp"], tauddmc=w.tauddmc, taulump=w.taulump, **kw)
+
+
+def test_leakage_emitprob_vs_reference(c1, golden):
+    w, st, wl, lines = c1
+    g = golden("ref_c1_opacity.npz")
+    m = _model(st, w, wl, g)
+    ol = m.leakage()
+    for k in (0, 1, 6, 7, 9, 10):
+        ref = g["opaclump"][:, k]
+        np.testing.assert_allclose(ol[:, k], ref, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(m.emission_probability(), g["emitprob"], rtol=1e-13)
+    emit, _ = po.sourceenergy(st["rcell"], st["temp"], w.lbol)
+    np.testing.assert_allclose(emit, g["emit"], rtol=1e-14)
+
+
+def test_emitgroup_vs_reference(c1, golden):
+    w, st, wl, lines = c1
+    g = golden("ref_c1_opacity.npz")
+    m = _model(st, w, wl, g, opaclump=g["opaclump"], emitprob=g["emitprob"], emit=g["emit"])
+    got = np.array([m.emitgroup(r, int(ic)) for r, ic in zip(g["eg_r"], g["eg_ic"])])
+    assert np.array_equal(got, g["eg_ig"])
+    # H7: with ng=1000, nep=32, nepg=31, groups above 992 are never re-emitted
+    assert got.max() <= 992
+
+
+def test_sourcenumbers_invariants(c1, golden):
+    w, st, wl, lines = c1
+    g = golden("ref_c1_opacity.npz")
+    for n in (1000, 4096, 1 << 20):
+        nvol = po.sourcenumbers(g["emit"], n)
+        assert nvol.sum() == n
+        assert np.all(nvol >= 1)
+        frac = nvol / n - g["emit"] / g["emit"].sum()
+        assert np.max(np.abs(frac)) <= 2.0 / n + 1e-12
+
+
+# --- transport statistics vs SuperLite's own Monte Carlo ----------------------------
+def _zstats(ref_rows, our_rows):
+    a, b = np.asarray(ref_rows, float), np.asarray(our_rows, float)
+    ma, sa = a.mean(0), a.std(0, ddof=1) / np.sqrt(len(a))
+    mb, sb = b.mean(0), b.std(0, ddof=1) / np.sqrt(len(b))
+    s = np.sqrt(sa ** 2 + sb ** 2)
+    ok = s > 0
+    return (ma[ok] - mb[ok]) / s[ok]
+
+
+def coarse(a, nb=20):
+    a = np.asarray(a)
+    n = a.shape[-1] - a.shape[-1] % nb
+    return a[..., :n].reshape(a.shape[:-1] + (nb, -1)).sum(-1)
+
+
+@pytest.mark.parametrize("case", ["c1mixed", "c1"])
+def test_oracle_transport_vs_reference_statistics(c1, golden, case):
+    w, st, wl, lines = c1
+    g = golden(f"ref_{case}_mc.npz")
+    m = _model(st, w, wl, g, opaclump=g["opaclump"], emitprob=g["emitprob"], emit=g["emit"],
+               nvol=po.sourcenumbers(g["emit"], 1 << int(g["n2s"])))
+    rows = [m.iteration(seed=11, iteration=it + 1) for it in range(6)]
+    for it, r in enumerate(rows):
+        # energy conservation of every realisation
+        assert r["estart"] == pytest.approx(r["eout"] + r["evelo"] + r["edead"], rel=1e-12)
+        assert r["nerr"] == 0
+    for key, f in (("luminos", coarse), ("tally", lambda a: coarse(a, 10))):
+        z = _zstats(f(g[key]), [f(r[key]) for r in rows])
+        assert np.mean(z ** 2) < 2.5, (key, np.round(z, 2))
+        assert np.max(np.abs(z)) < 4.5, (key, np.round(z, 2))
+    for key in ("eout", "evelo", "nstepimc", "nstepddmc"):
+        z = _zstats(g[key][:, None], [[r[key]] for r in rows])
+        assert abs(z[0]) < 4.0, (key, z)
