@@ -55,7 +55,7 @@ typedef struct {
     int precision;         /* SLGPU_FP64 or SLGPU_MIXED */
     uint32_t seed;         /* key of the counter-based per-particle RNG (replaces randommod.f) */
     int64_t npartmax;      /* prt_npartmax (particlemod.f90:175) */
-    int64_t max_steps;     /* per-particle step cap, a particle beyond it dies with error 200 (0: 2^40) */
+    int64_t max_steps;     /* per-particle step cap, a particle beyond it dies with error 200 (0: 2^26) */
 } slgpu_config;
 
 /* Host gas state consumed by the opacity build (gasmod.f:20-34 and ion_grndlev,

@@ -819,6 +819,11 @@ static int or_transport11(const or_model *m, or_ptcl *p, double *vx, or_stream *
             wl = 1.0 / ((1.0 - r1) * WLINV(ig) + r1 * WLINV(ig + 1));
             wl = wl * (1.0 - (*vx) * mu * CINV);
         }
+    } else if (d == db && ixnext > ix && ix == nx) {
+        /* outer boundary reached with a rounding-negative cosine: escapes (the reference
+         * would index the non-existent cell nx+1) */
+        p->stat = 1;
+        goto done;
     } else if (d == db && ixnext > ix) {
         int l = ix + 1;
         if ((SIG(l) + CAP(ig, l)) * DX(ix + 1) < m->tauddmc || m->puretran) {

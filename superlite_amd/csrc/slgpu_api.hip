@@ -146,7 +146,7 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     d.taulump = cfg->taulump;
     d.puretran = cfg->puretran;
     d.precision = cfg->precision;
-    d.max_steps = cfg->max_steps > 0 ? cfg->max_steps : ((int64_t)1 << 40);
+    d.max_steps = cfg->max_steps > 0 ? cfg->max_steps : ((int64_t)1 << 26);
     d.seed = cfg->seed;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -441,7 +441,7 @@ int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *o
     HIPCHK(hipMemcpyAsync(c->d.temp, gas->temp, nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d.mass, gas->mass, nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d.natomfr, gas->natomfr, (size_t)nc * 30 * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d.el_z, elz, nel * sizeof(int) + 4, hipMemcpyHostToDevice, c->stream));
+    if (nel > 0) HIPCHK(hipMemcpyAsync(c->d.el_z, elz, nel * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(dn, gas->natom, nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(dv, gas->vol, nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(de, gas->nelec, nc * 8, hipMemcpyHostToDevice, c->stream));

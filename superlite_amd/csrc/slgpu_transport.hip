@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void k_sources(Dev d, uint32_t iteration)
         r2 = rng.draw();
         ++guard;
     }
-    const int ix = binsrch(x0, d.xarr, nx + 1, false);
+    int ix = binsrch(x0, d.xarr, nx + 1, false);
+    ix = ix < 1 ? i : ix;
     const double h1 = XA(ix), h2 = XA(ix + 1);
     const double vx = (VXA(ix) * (h2 - x0) + VXA(ix + 1) * (x0 - h1)) / (h2 - h1);
     const double cmffact = 1.0 + mu0 * vx * CINV;
@@ -245,6 +246,11 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             wl = 1.0 / ((1.0 - r1) * WLI(ig) + r1 * WLI(ig + 1));
             wl = wl * (1.0 - vx * mu * CINV);
         }
+    } else if (dd == db && ixnext > ix && ix == nx) {
+        // outer boundary reached with a rounding-negative cosine: escapes (the reference
+        // would index the non-existent cell nx+1 here)
+        p.stat = 1;
+        goto done;
     } else if (dd == db && ixnext > ix) {
         const int l = ix + 1;
         if ((SIGT(l) + CAPT(ig, l)) * DXX(ix + 1) < d.tauddmc || d.puretran) {
@@ -614,6 +620,8 @@ __global__ __launch_bounds__(256) void k_transport(Dev d, uint32_t iteration)
             esrc += p.e;
             p.ix = binsrch(p.x, d.xarr, nx + 1, false);
             p.ig = binsrch(p.wl, d.wl, ng + 1, false);
+            bool bad = p.ix < 1 || p.ig < 1;     // NaN position or wavelength
+            if (bad) { p.ix = 1; p.ig = 1; }
             const double help = DXX(p.ix);
             const double tau = (SIGT(p.ix) + CAPT(p.ig, p.ix)) * help;
             p.itype = (d.puretran || tau < d.tauddmc) ? 1 : 2;
@@ -632,6 +640,16 @@ __global__ __launch_bounds__(256) void k_transport(Dev d, uint32_t iteration)
             istep = 0;
             nact++;
             have = true;
+            if (bad) {
+                nerr++;
+                ndead++;
+                edead += p.e;
+                d.pstat[ip] = 2;
+                d.perr[ip] = 1;
+                d.pnstep[ip] = 0;
+                have = false;
+                continue;
+            }
         }
         // one event
         istep++;
@@ -724,6 +742,7 @@ __global__ __launch_bounds__(256) void k_fluxtally(Dev d)
     if (p >= d.npart) return;
     if (d.pstat[p] != 1) return;
     const int ig = binsrch(d.pwl[p], d.wl, d.ng + 1, false);
+    if (ig < 1) return;                  // NaN wavelength: not binned
     const double e = d.pe[p];
     atomicAdd(&d.lum[ig - 1], e);
     atomicAdd(&d.dev[ig - 1], e * e);

@@ -223,7 +223,7 @@ def test_transport_statistics_vs_reference(S, tables, golden, case):
     st["nelec"] = st["nelec"] * scale
     ctx = S.setup_workload(w, st, wl, lines, tables, npartmax=1 << n2s, seed=21)
     rows = []
-    for it in range(12):
+    for it in range(24):
         r = S.iteration(ctx, st, 1 << n2s, it + 1, w.lbol, flags=int(g["flags"]))
         rows.append(dict(luminos=r["flux"]["luminos"], tally=r["transport"]["tally"],
                          eout=r["transport"]["eout"], evelo=r["transport"]["evelo"]))
