@@ -15,20 +15,20 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-munsafe-fp-ato
          "-Wno-unused-variable", "-Wno-unused-function", "-Wno-unused-value"]
 
 
-def build(verbose: bool = False) -> str:
-    objdir = os.path.join(HERE, "_build")
+def build(verbose: bool = False, extra=(), out: str = LIB) -> str:
+    objdir = os.path.join(os.path.dirname(out), "_build_" + os.path.basename(out).replace(".", "_"))
     os.makedirs(objdir, exist_ok=True)
     objs = []
     procs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
     failed = False
     for cmd, p in procs:
-        out, _ = p.communicate()
-        text = out.decode(errors="replace")
+        pout, _ = p.communicate()
+        text = pout.decode(errors="replace")
         text = "\n".join(l for l in text.splitlines() if "packed-fp32-ops" not in l)
         if p.returncode != 0:
             failed = True
@@ -37,11 +37,11 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(text + "\n")
     if failed:
         raise RuntimeError("hipcc failed")
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -189,6 +189,8 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.cumR, cg);
     rc |= dalloc(c, &d.cumD, cg);
     rc |= dalloc(c, &d.cumS, cg);
+    rc |= dalloc(c, &d.guide, (size_t)4 * nc * GUIDE_K);
+    rc |= dalloc(c, &d.guide_m, (size_t)4 * nc);
     rc |= dalloc(c, (int64_t **)&d.pstart, nc + 1);
     rc |= dalloc(c, &d.px, np);
     rc |= dalloc(c, &d.pmu, np);

@@ -275,6 +275,25 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
         if (p < ng) cS[p] = carS + sS;
         carS += tot;
     }
+    __syncthreads();
+    // guide tables of the four cumulative rows (the effective-scatter row starts at nlump)
+    for (int t = 0; t < 4; ++t) {
+        const double *c = t == 0 ? cL : (t == 1 ? cR : (t == 2 ? cD : cS + nl_eff));
+        const int n = t == 3 ? ng - nl_eff : nl_eff;
+        int *G = d.guide + ((size_t)t * nx + (l - 1)) * GUIDE_K;
+        const double last = n > 0 ? c[n - 1] : 0.0;
+        const double m = last > 0.0 ? (double)GUIDE_K / last : 0.0;
+        if (threadIdx.x == 0) d.guide_m[t * nx + (l - 1)] = m;
+        for (int b = threadIdx.x; b < GUIDE_K; b += NT) {
+            int lo = 0, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (guide_key(c[mid], m) >= b) hi = mid;
+                else lo = mid + 1;
+            }
+            G[b] = lo;
+        }
+    }
 }
 
 // emission_probability.f90:39-62 and the in-bin cumulative sums emitgroup.f:278-294 walks

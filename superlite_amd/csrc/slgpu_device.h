@@ -140,7 +140,9 @@ struct Philox {
                                                  uint32_t c3, uint32_t k0, uint32_t k1,
                                                  uint32_t out[4])
     {
-        #pragma unroll
+        // rolled: one copy of the round per draw site keeps the transport kernel's code
+        // small enough for the instruction cache
+        #pragma unroll 1
         for (int r = 0; r < 10; ++r) {
             uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
             uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
@@ -170,6 +172,27 @@ struct Philox {
         return u01(w2, w3);
     }
 };
+
+// Guide table of a nondecreasing cumulative row c[0..n): with key(v) = min(K-1, (int)(v*m)),
+// guide[b] = first j with key(c[j]) >= b.  For a query r the first j with c[j] > r is never
+// below guide[key(r)] (key is monotone), so a forward scan from there returns exactly the
+// index a binary search would, after an expected 1 + n/K probes of one contiguous row.
+constexpr int GUIDE_K = 2048;
+
+__device__ __forceinline__ int guide_key(double v, double m)
+{
+    double k = v * m;
+    return k >= (double)(GUIDE_K - 1) ? GUIDE_K - 1 : (k > 0.0 ? (int)k : 0);
+}
+
+__device__ __forceinline__ int guided_first_above(const double *__restrict__ c, int n,
+                                                  const int *__restrict__ guide, double m,
+                                                  double r)
+{
+    int j = guide[guide_key(r, m)];
+    while (j < n && !(c[j] > r)) ++j;
+    return j;
+}
 
 // wave64 inclusive scan of doubles (fixed order: result is deterministic)
 __device__ __forceinline__ double wave_inclusive_scan(double v)

@@ -62,6 +62,8 @@ struct Dev {
     CellDDMC *cell;             // nc
     int *gorder;                // nc*ng  lumped groups ascending, then unlumped ascending
     double *cumL, *cumR, *cumD, *cumS;   // nc*ng cumulative lumped sampling weights
+    int *guide;                 // 4*nc*GUIDE_K guide tables of cumL, cumR, cumD, cumS
+    double *guide_m;            // 4*nc key multipliers
     // particles (SoA)
     int64_t npart;
     const int64_t *pstart;      // nc+1

@@ -436,7 +436,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         else {
             r1 = s.draw();
             const int n = cc.nlump;
-            int j = first_above(d.cumD + row, 0, n, r1);
+            int j = guided_first_above(d.cumD + row, n, d.guide + ((size_t)2 * nx + ic - 1) * GUIDE_K, d.guide_m[2 * nx + ic - 1], r1);
             if (DIVV(ix) >= 0) iiig = d.gorder[row + (j < n ? j : n - 1)];
             else iiig = d.gorder[row + (j < n ? n - 1 - j : 0)];
         }
@@ -472,7 +472,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         else {
             r1 = s.draw();
             const int n = cc.nlump;
-            int j = first_above(d.cumL + row, 0, n, r1);
+            int j = guided_first_above(d.cumL + row, n, d.guide + ((size_t)0 * nx + ic - 1) * GUIDE_K, d.guide_m[0 * nx + ic - 1], r1);
             iiig = d.gorder[row + (j < n ? j : n - 1)];
         }
         if ((SIGT(l) + CAPT(iiig, l)) * DXX(ix - 1) < d.tauddmc) {
@@ -504,7 +504,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             if (glump != 0) {
                 r1 = s.draw();
                 const int n = cc.nlump;
-                int j = first_above(d.cumR + row, 0, n, r1);
+                int j = guided_first_above(d.cumR + row, n, d.guide + ((size_t)1 * nx + ic - 1) * GUIDE_K, d.guide_m[1 * nx + ic - 1], r1);
                 ig = d.gorder[row + (j < n ? j : n - 1)];
             }
             r1 = s.draw();
@@ -525,7 +525,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         else {
             r1 = s.draw();
             const int n = cc.nlump;
-            int j = first_above(d.cumR + row, 0, n, r1);
+            int j = guided_first_above(d.cumR + row, n, d.guide + ((size_t)1 * nx + ic - 1) * GUIDE_K, d.guide_m[1 * nx + ic - 1], r1);
             iiig = d.gorder[row + (j < n ? j : n - 1)];
         }
         if ((SIGT(l) + CAPT(iiig, l)) * DXX(ix + 1) < d.tauddmc) {
@@ -556,7 +556,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         } else {
             r1 = s.draw();
             const int n = cc.nlump;
-            int j = first_above(d.cumS + row, n, ng, r1);
+            int j = n + guided_first_above(d.cumS + row + n, ng - n, d.guide + ((size_t)3 * nx + ic - 1) * GUIDE_K, d.guide_m[3 * nx + ic - 1], r1);
             iiig = d.gorder[row + (j < ng ? j : ng - 1)];
         }
         ig = iiig;
@@ -594,7 +594,11 @@ ret_early:
 }
 
 // particle_advance.f90:140-294 as a persistent kernel; see the file header.
-__global__ __launch_bounds__(256) void k_transport(Dev d, uint32_t iteration)
+#ifndef SLGPU_TRANSPORT_WAVES
+#define SLGPU_TRANSPORT_WAVES 2
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SLGPU_TRANSPORT_WAVES, 8)))
+void k_transport(Dev d, uint32_t iteration)
 {
     extern __shared__ double s_tally[];
     const int ng = d.ng, nx = d.nx;
