@@ -136,6 +136,16 @@ int slgpu_fluxtally(slgpu_ctx *ctx, double *out_luminos, double *out_lumdev,
 int slgpu_get_particles(slgpu_ctx *ctx, int64_t n, uint8_t *stat, double *x, double *mu,
                         double *wl, double *e, int *ig, int64_t *nstep, int *err);
 
+/* Verification hook for the transport's group samplers; no single reference routine
+ * corresponds.  It evaluates, for n (draw r, cell) pairs, the sampler the transport
+ * uses: table 0-3 = the lumped loops of diffusion11.f90 (left leak :375-398, right leak
+ * :447-460/489-511, Doppler :289-313, effective scatter :571-584), table 4 = emitgroup.f:23-51.
+ * direct = 0 goes through the bucket maps the kernels use, direct = 1 through the plain
+ * searches they replace; both must agree for every r.  Requires slgpu_ddmc_prep.
+ * Lumped tables of cells without lumped groups return 0.  Host arrays. */
+int slgpu_sample_groups(slgpu_ctx *ctx, int table, int direct, int64_t n, const double *r,
+                        const int *cell, int *out_group);
+
 /* accumulated device time per phase in ms, measured with HIP events on the context
  * stream: [0] opacity, [1] ddmc prep, [2] sources, [3] transport, [4] fluxtally,
  * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone; reset = 1 zeroes them */

@@ -10,6 +10,7 @@
 
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
+#include "slgpu_sample.h"
 
 using namespace slgpu;
 
@@ -184,13 +185,14 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.radtemp, nc);
     rc |= dalloc(c, &d.srccum, cg);
     rc |= dalloc(c, &d.cell, nc);
+    rc |= dalloc(c, &d.cprob, nc);
+    rc |= dalloc(c, &d.gprob, cg);
     rc |= dalloc(c, &d.gorder, cg);
     rc |= dalloc(c, &d.cumL, cg);
     rc |= dalloc(c, &d.cumR, cg);
     rc |= dalloc(c, &d.cumD, cg);
     rc |= dalloc(c, &d.cumS, cg);
-    rc |= dalloc(c, &d.guide, (size_t)4 * nc * GUIDE_K);
-    rc |= dalloc(c, &d.guide_m, (size_t)4 * nc);
+    rc |= dalloc(c, &d.btab, (size_t)NTAB * nc * BT_K);
     rc |= dalloc(c, (int64_t **)&d.pstart, nc + 1);
     rc |= dalloc(c, &d.px, np);
     rc |= dalloc(c, &d.pmu, np);
@@ -201,8 +203,17 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.pig, np);
     rc |= dalloc(c, &d.perr, np);
     rc |= dalloc(c, &d.pnstep, np);
+    rc |= dalloc(c, &d.pix, np);
     rc |= dalloc(c, &d.tally, nc);
-    rc |= dalloc(c, &d.jrad, cg);
+    // grd_jrad gets one atomic per event; on small grids every lane hits a few hot
+    // (cell, group) entries, so blocks add into one of R replicas (R from a 256 MiB budget,
+    // 1..32) that a reduction folds into replica 0 after the transport
+    {
+        const size_t budget = (size_t)256 << 20;
+        size_t r = budget / (cg * sizeof(double));
+        d.jrad_copies = (int)(r < 1 ? 1 : (r > 32 ? 32 : r));
+    }
+    rc |= dalloc(c, &d.jrad, (size_t)d.jrad_copies * cg);
     rc |= dalloc(c, &d.tot, 8);
     rc |= dalloc(c, &d.ct, 10);
     rc |= dalloc(c, &d.queue, 1);
@@ -497,6 +508,7 @@ int slgpu_ddmc_prep(slgpu_ctx *c, const double *radtemp, double lbol, double *ou
     HIPCHK(hipMemcpyAsync(c->d.emit, emit.data(), nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d.radtemp, radtemp, nc * 8, hipMemcpyHostToDevice, c->stream));
     launch_ddmc_tables(c);
+    launch_ddmc_probs(c);
     HIPCHK(hipGetLastError());
     if (out_opaclump) HIPCHK(hipMemcpyAsync(out_opaclump, c->d.opaclump, (size_t)11 * nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_emitprob) HIPCHK(hipMemcpyAsync(out_emitprob, c->d.emitprob, (size_t)c->cfg.nep * nc * 8, hipMemcpyDeviceToHost, c->stream));
@@ -549,12 +561,13 @@ int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double 
     PhaseTimer tm(c, 3);
     // particle_advance.f90:65-67: zero the tallies
     HIPCHK(hipMemsetAsync(c->d.tally, 0, nc * 8, c->stream));
-    HIPCHK(hipMemsetAsync(c->d.jrad, 0, (size_t)nc * ng * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.jrad, 0, (size_t)c->d.jrad_copies * nc * ng * 8, c->stream));
     HIPCHK(hipMemsetAsync(c->d.tot, 0, 8 * 8, c->stream));
     HIPCHK(hipMemsetAsync(c->d.ct, 0, 10 * 8, c->stream));
-    HIPCHK(hipMemsetAsync(c->d.queue, 0, 8, c->stream));
     hipEventRecord(c->ev[12], c->stream);
+    HIPCHK(hipMemsetAsync(c->d.queue, 0, 8, c->stream));
     launch_transport(c, iteration);
+    launch_jrad_reduce(c);
     hipEventRecord(c->ev[13], c->stream);
     HIPCHK(hipGetLastError());
     double tot[8];
@@ -613,6 +626,36 @@ int slgpu_get_particles(slgpu_ctx *c, int64_t n, uint8_t *stat, double *x, doubl
     if (err) HIPCHK(hipMemcpyAsync(err, c->d.perr, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int slgpu_sample_groups(slgpu_ctx *c, int table, int direct, int64_t n, const double *r,
+                        const int *cell, int *out_group)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_prep) return fail(SLGPU_EORDER, "slgpu_ddmc_prep must run first");
+    if (table < 0 || table >= NTAB) return fail(SLGPU_EINVAL, "table must be in 0..4");
+    if (n < 0 || (n > 0 && (!r || !cell || !out_group))) return fail(SLGPU_EINVAL, "bad sample arrays");
+    if (n == 0) return 0;
+    for (int64_t k = 0; k < n; ++k)
+        if (!(r[k] > 0.0 && r[k] <= 1.0)) return fail(SLGPU_EINVAL, "draws must lie in (0, 1]");
+    double *dr = nullptr;
+    int *dc = nullptr, *dg = nullptr;
+    if (hipMalloc(&dr, n * 8) != hipSuccess || hipMalloc(&dc, n * 4) != hipSuccess ||
+        hipMalloc(&dg, n * 4) != hipSuccess) {
+        hipFree(dr); hipFree(dc); hipFree(dg);
+        return fail(SLGPU_ENOMEM, "sample buffers");
+    }
+    int rc = 0;
+    if (hipMemcpyAsync(dr, r, n * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dc, cell, n * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+    if (!rc) {
+        launch_sample_groups(c, table, direct, n, dr, dc, dg);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out_group, dg, n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) rc = 1;
+    }
+    hipFree(dr); hipFree(dc); hipFree(dg);
+    return rc ? fail(SLGPU_EHIP, "sample_groups: device error") : 0;
 }
 
 int slgpu_timers(slgpu_ctx *c, double *ms, int n, int reset)

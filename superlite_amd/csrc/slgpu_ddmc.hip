@@ -2,11 +2,13 @@
 // binned emission probabilities (GRID/emission_probability.f90:39-62), and the per-cell
 // sampling tables that replace the reference's per-event O(ng) linear scans
 // (GRID/emitgroup.f:23-51, TRANSPORT1/diffusion11.f90:289-313, 375-398, 447-460,
-// 489-511, 571-584; SOURCE/interior_source.f90:96-103) by binary searches in the
-// transport and source kernels.  One 1024-thread block per cell; groups are swept in
-// chunks of 1024 with block-wide (wave64 shuffle) scans carrying the running prefix.
+// 489-511, 571-584; SOURCE/interior_source.f90:96-103) by cumulative rows plus exact
+// bucket maps (slgpu_sample.h), so a sampled group usually costs one table load.  One
+// 1024-thread block per cell; groups are swept in chunks of 1024 with block-wide (wave64
+// shuffle) scans carrying the running prefix.
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
+#include "slgpu_sample.h"
 
 namespace slgpu {
 
@@ -57,6 +59,37 @@ __device__ __forceinline__ int block_scan_int(int v, int *sh, int &total)
     total = sh[NW - 1];
     __syncthreads();
     return x + off;
+}
+
+// in-place running sum of row[0..n) in index order (loads batched, adds sequential)
+__device__ void running_sum(double *row, int n)
+{
+    double s = 0.0;
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        double v[8];
+        #pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = row[j + k];
+        #pragma unroll
+        for (int k = 0; k < 8; ++k) { s = s + v[k]; row[j + k] = s; }
+    }
+    for (; j < n; ++j) { s = s + row[j]; row[j] = s; }
+}
+
+// sum of row[0..n) in index order
+__device__ double sequential_sum(const double *row, int n)
+{
+    double s = 0.0;
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        double v[8];
+        #pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = row[j + k];
+        #pragma unroll
+        for (int k = 0; k < 8; ++k) s = s + v[k];
+    }
+    for (; j < n; ++j) s = s + row[j];
+    return s;
 }
 
 __device__ __forceinline__ double block_sum(double v, double *sh)
@@ -234,66 +267,42 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
         baseU += tu;
     }
     __syncthreads();
-    const double hL = 1.0 / o1, hR = 1.0 / o2, hD = 1.0 / c_doplump;
+    // a face or Doppler channel with zero total opacity is never chosen (its event
+    // probability is 0); its row is kept finite (all zero) so every row is monotone
+    const double hL = o1 > 0.0 ? 1.0 / o1 : 0.0, hR = o2 > 0.0 ? 1.0 / o2 : 0.0;
+    const double hD = c_doplump > 0.0 ? 1.0 / c_doplump : 0.0;
     const double d2s = 1.0 / (1.0 - c_emitlump);
     double *cL = d.cumL + (size_t)(l - 1) * ng, *cR = d.cumR + (size_t)(l - 1) * ng;
     double *cD = d.cumD + (size_t)(l - 1) * ng, *cS = d.cumS + (size_t)(l - 1) * ng;
-    double carL = 0.0, carR = 0.0, carD = 0.0;
-    for (int p0 = 0; p0 < nl_eff; p0 += NT) {
-        const int p = p0 + threadIdx.x;
-        double tL = 0.0, tR = 0.0, tD = 0.0;
-        if (p < nl_eff) {
-            // the Doppler scan runs ascending for redshift, descending for blueshift
-            const int pd = divv >= 0 ? p : nl_eff - 1 - p;
-            const int g = gorder[p], gd = gorder[pd];
-            const double sp = spec[g - 1];
-            if (i > 1) tL = sp * resleak(d, i, g, -1) * speclump * hL;
-            tR = sp * resleak(d, i, g, +1) * speclump * hR;
-            if (divv >= 0) {
-                if (gd != ng && !(CAPC(gd + 1, l) * dist >= d.taulump))
-                    tD = dopspeccalc(tinv, d.wlinv, gd) * divv / (3 * PC_C) * speclump * hD;
-            } else {
-                if (gd != 1 && !(CAPC(gd - 1, l) * dist >= d.taulump))
-                    tD = -(dopspeccalc(tinv, d.wlinv, gd - 1) * divv / (3 * PC_C) * speclump * hD);
-            }
+    // terms in parallel, then each row's running sum in index order by one lane: the
+    // reference accumulates sequentially, and a sequential sum of nonnegative terms is
+    // monotone (a parallel scan is not, by rounding), which the samplers' searches need
+    for (int p = threadIdx.x; p < nl_eff; p += NT) {
+        // the Doppler scan runs ascending for redshift, descending for blueshift
+        const int pd = divv >= 0 ? p : nl_eff - 1 - p;
+        const int g = gorder[p], gd = gorder[pd];
+        const double sp = spec[g - 1];
+        double tD = 0.0;
+        cL[p] = i > 1 ? sp * resleak(d, i, g, -1) * speclump * hL : 0.0;
+        cR[p] = sp * resleak(d, i, g, +1) * speclump * hR;
+        if (divv >= 0) {
+            if (gd != ng && !(CAPC(gd + 1, l) * dist >= d.taulump))
+                tD = dopspeccalc(tinv, d.wlinv, gd) * divv / (3 * PC_C) * speclump * hD;
+        } else {
+            if (gd != 1 && !(CAPC(gd - 1, l) * dist >= d.taulump))
+                tD = -(dopspeccalc(tinv, d.wlinv, gd - 1) * divv / (3 * PC_C) * speclump * hD);
         }
-        double tot;
-        double sL = block_scan(tL, shd, tot); if (p < nl_eff) cL[p] = carL + sL; carL += tot;
-        double sR = block_scan(tR, shd, tot); if (p < nl_eff) cR[p] = carR + sR; carR += tot;
-        double sD = block_scan(tD, shd, tot); if (p < nl_eff) cD[p] = carD + sD; carD += tot;
+        cD[p] = tD;
     }
-    double carS = 0.0;
-    for (int p0 = nl_eff; p0 < ng; p0 += NT) {
-        const int p = p0 + threadIdx.x;
-        double tS = 0.0;
-        if (p < ng) {
-            const int g = gorder[p];
-            tS = spec[g - 1] * CAPC(g, l) * cgi * d2s;
-        }
-        double tot;
-        double sS = block_scan(tS, shd, tot);
-        if (p < ng) cS[p] = carS + sS;
-        carS += tot;
+    for (int p = nl_eff + threadIdx.x; p < ng; p += NT) {
+        const int g = gorder[p];
+        cS[p] = spec[g - 1] * CAPC(g, l) * cgi * d2s;
     }
     __syncthreads();
-    // guide tables of the four cumulative rows (the effective-scatter row starts at nlump)
-    for (int t = 0; t < 4; ++t) {
-        const double *c = t == 0 ? cL : (t == 1 ? cR : (t == 2 ? cD : cS + nl_eff));
-        const int n = t == 3 ? ng - nl_eff : nl_eff;
-        int *G = d.guide + ((size_t)t * nx + (l - 1)) * GUIDE_K;
-        const double last = n > 0 ? c[n - 1] : 0.0;
-        const double m = last > 0.0 ? (double)GUIDE_K / last : 0.0;
-        if (threadIdx.x == 0) d.guide_m[t * nx + (l - 1)] = m;
-        for (int b = threadIdx.x; b < GUIDE_K; b += NT) {
-            int lo = 0, hi = n;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (guide_key(c[mid], m) >= b) hi = mid;
-                else lo = mid + 1;
-            }
-            G[b] = lo;
-        }
-    }
+    if (threadIdx.x == 0) running_sum(cL, nl_eff);
+    else if (threadIdx.x == 64) running_sum(cR, nl_eff);
+    else if (threadIdx.x == 128) running_sum(cD, nl_eff);
+    else if (threadIdx.x == 192) running_sum(cS + nl_eff, ng - nl_eff);
 }
 
 // emission_probability.f90:39-62 and the in-bin cumulative sums emitgroup.f:278-294 walks
@@ -336,32 +345,62 @@ __global__ __launch_bounds__(256) void k_emitprob(Dev d)
     }
 }
 
-// normalised Planck at T_rad and its cumulative (interior_source.f90:69-75, 96-103)
+// normalised Planck at T_rad and its cumulative (interior_source.f90:69-75, 96-103):
+// terms in parallel; the normalising sum and the running sum in group order, as the
+// reference's loops do
 __global__ __launch_bounds__(NT) void k_srccum(Dev d)
 {
-    __shared__ double shd[NW];
+    __shared__ double s_norm;
     const int ng = d.ng;
     const int l = blockIdx.x;
     const double tr = 1.0 / d.radtemp[l];
-    double s = 0.0;
-    for (int g = threadIdx.x + 1; g <= ng; g += NT) s += specintv_g(tr, d.wlinv, g);
-    s = block_sum(s, shd);
     double *cum = d.srccum + (size_t)l * ng;
-    double car = 0.0;
-    for (int g0 = 1; g0 <= ng; g0 += NT) {
-        const int g = g0 + threadIdx.x;
-        const double p = g <= ng ? specintv_g(tr, d.wlinv, g) / s : 0.0;
-        double tot;
-        double x = block_scan(p, shd, tot);
-        if (g <= ng) cum[g - 1] = car + x;
-        car += tot;
+    for (int g = threadIdx.x + 1; g <= ng; g += NT) cum[g - 1] = specintv_g(tr, d.wlinv, g);
+    __syncthreads();
+    if (threadIdx.x == 0) s_norm = sequential_sum(cum, ng);
+    __syncthreads();
+    const double s = s_norm;
+    for (int g = threadIdx.x + 1; g <= ng; g += NT) cum[g - 1] = cum[g - 1] / s;
+    __syncthreads();
+    if (threadIdx.x == 0) running_sum(cum, ng);
+}
+
+// exact bucket maps of the five group samplers; one thread per (bucket, cell, table)
+__global__ __launch_bounds__(256) void k_bucket_tables(Dev d)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int ic = blockIdx.y + 1, t = blockIdx.z;
+    if (b >= BT_K) return;
+    uint32_t *row = d.btab + ((size_t)t * d.nx + (ic - 1)) * BT_K;
+    const double rlo = bt_lo(b), rhi = bt_hi(b);
+    uint32_t e = 0;
+    if (t == TAB_E) {
+        const int glo = emitgroup_direct(d, rlo, ic), ghi = emitgroup_direct(d, rhi, ic);
+        if (glo == ghi) e = BT_PURE | (dest_thin(d, t, ic, glo) ? BT_THIN : 0u) | (uint32_t)glo;
+        else e = ((uint32_t)ghi << 15) | (uint32_t)glo;
+    } else {
+        const LumpRow L = lump_row(d, t, ic, d.cell[ic - 1].nlump);
+        if (L.cnt > 0) {                    // rows of cells without lumping are never sampled
+            const int jlo = first_above(L.cum, 0, L.cnt, rlo);
+            const int jhi = first_above(L.cum, 0, L.cnt, rhi);
+            const int plo = jlo < L.cnt ? jlo : L.cnt - 1, phi = jhi < L.cnt ? jhi : L.cnt - 1;
+            if (plo == phi) {
+                const int g = lump_group(d, L, ic, jlo);
+                e = BT_PURE | (dest_thin(d, t, ic, g) ? BT_THIN : 0u) | (uint32_t)g;
+            } else {
+                e = ((uint32_t)jhi << 15) | (uint32_t)jlo;
+            }
+        }
     }
+    row[b] = e;
 }
 
 void launch_ddmc_tables(slgpu_ctx *c)
 {
     hipLaunchKernelGGL(k_leakage, dim3(c->nx), dim3(NT), 0, c->stream, c->d);
     hipLaunchKernelGGL(k_emitprob, dim3(c->nx), dim3(256), 0, c->stream, c->d);
+    hipLaunchKernelGGL(k_bucket_tables, dim3(BT_K / 256, c->nx, NTAB), dim3(256), 0, c->stream,
+                       c->d);
     hipLaunchKernelGGL(k_srccum, dim3(c->nx), dim3(NT), 0, c->stream, c->d);
 }
 

@@ -160,6 +160,17 @@ struct Philox {
         return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
     }
 
+    // continue a stream after n draws (re-derives the buffered half block when n is odd)
+    __device__ __forceinline__ void resume(uint32_t n)
+    {
+        ndraw = n;
+        if (n & 1u) {
+            uint32_t o[4];
+            block(pid, n >> 1, phase, 0u, k0, k1, o);
+            w2 = o[2]; w3 = o[3];
+        }
+    }
+
     __device__ __forceinline__ double draw()
     {
         uint32_t k = ndraw++;
@@ -172,27 +183,6 @@ struct Philox {
         return u01(w2, w3);
     }
 };
-
-// Guide table of a nondecreasing cumulative row c[0..n): with key(v) = min(K-1, (int)(v*m)),
-// guide[b] = first j with key(c[j]) >= b.  For a query r the first j with c[j] > r is never
-// below guide[key(r)] (key is monotone), so a forward scan from there returns exactly the
-// index a binary search would, after an expected 1 + n/K probes of one contiguous row.
-constexpr int GUIDE_K = 2048;
-
-__device__ __forceinline__ int guide_key(double v, double m)
-{
-    double k = v * m;
-    return k >= (double)(GUIDE_K - 1) ? GUIDE_K - 1 : (k > 0.0 ? (int)k : 0);
-}
-
-__device__ __forceinline__ int guided_first_above(const double *__restrict__ c, int n,
-                                                  const int *__restrict__ guide, double m,
-                                                  double r)
-{
-    int j = guide[guide_key(r, m)];
-    while (j < n && !(c[j] > r)) ++j;
-    return j;
-}
 
 // wave64 inclusive scan of doubles (fixed order: result is deterministic)
 __device__ __forceinline__ double wave_inclusive_scan(double v)

@@ -26,6 +26,13 @@ struct CellDDMC {
     int pad;
 };
 
+// Event thresholds of one DDMC step in a cell (lumped) or a (cell, group) (unlumped),
+// diffusion11.f90:243-283: 1/denominator and the cumulative Doppler / left / right
+// probabilities the event draw is compared against.
+struct DdmcProb {
+    double denom, t1, t2, t3;
+};
+
 // Everything the kernels read or write, passed to kernels by value.
 struct Dev {
     int nx, ng, nep, nepg, nion, nel;
@@ -60,19 +67,21 @@ struct Dev {
     double *radtemp;            // nc
     double *srccum;             // nc*ng  cumulative normalised Planck at T_rad
     CellDDMC *cell;             // nc
+    DdmcProb *cprob;            // nc     lumped-group step of each cell
+    DdmcProb *gprob;            // nc*ng  unlumped step of each (cell, group), group-fastest
     int *gorder;                // nc*ng  lumped groups ascending, then unlumped ascending
     double *cumL, *cumR, *cumD, *cumS;   // nc*ng cumulative lumped sampling weights
-    int *guide;                 // 4*nc*GUIDE_K guide tables of cumL, cumR, cumD, cumS
-    double *guide_m;            // 4*nc key multipliers
+    uint32_t *btab;             // 5*nc*BT_K bucket maps of the cumL/R/D/S rows and emitgroup
     // particles (SoA)
     int64_t npart;
     const int64_t *pstart;      // nc+1
     double *px, *pmu, *pwl, *pe, *pe0;
     uint8_t *pstat;
-    int *pig, *perr;
+    int *pig, *pix, *perr;
     int64_t *pnstep;
     // tallies
-    double *tally, *jrad, *tot;           // nc, nc*ng, 8
+    double *tally, *jrad, *tot;           // nc, jrad_copies*nc*ng, 8
+    int jrad_copies;                      // replicas of grd_jrad that spread atomic traffic
     unsigned long long *ct;               // 10
     unsigned long long *queue;            // work counter of the persistent transport
     double *lum, *dev;                    // ng
@@ -118,7 +127,11 @@ void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
 void launch_opacity(slgpu_ctx *c, int flags);
 void launch_opacity_finish(slgpu_ctx *c);
 void launch_ddmc_tables(slgpu_ctx *c);
+void launch_ddmc_probs(slgpu_ctx *c);
 void launch_sources(slgpu_ctx *c, uint32_t iteration);
 void launch_transport(slgpu_ctx *c, uint32_t iteration);
 void launch_fluxtally(slgpu_ctx *c);
+void launch_jrad_reduce(slgpu_ctx *c);
+void launch_sample_groups(slgpu_ctx *c, int t, int direct, int64_t n, const double *r,
+                          const int *cell, int *out);
 }  // namespace slgpu

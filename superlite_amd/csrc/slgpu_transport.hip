@@ -5,19 +5,22 @@
 //                (one IMC event) and TRANSPORT1/diffusion11.f90:91-631 (one DDMC event)
 //   k_fluxtally  fluxtally.f:30-54
 //
-// The transport kernel is persistent: every lane owns one particle at a time and takes
-// one event per loop trip; a lane whose particle escaped or died fetches the next particle
-// index from a global work counter, so the SIMD stays full while particles with very
-// different event counts drain.  Every lane leaves the loop once the counter passes the
-// particle count and its own particle is finished (each particle is capped at max_steps).
+// Transport is a persistent kernel over SoA particle arrays: a lane runs one particle to
+// escape or death, then takes the next from a global counter, so lanes stay busy until
+// the pool is drained; every particle is capped at max_steps events.  The event count of
+// SuperLite's random walk (no census: a particle leaves only by escaping or dying) is
+// dominated by effective scatters, so the group samplers they call are the exact bucket
+// maps of slgpu_sample.h: one table load for most draws instead of nested searches.
 // Per-particle randomness is a Philox-4x32-10 stream keyed on (seed, iteration) and
 // counted on (particle, draw, phase), so trajectories do not depend on the schedule.
 //
-// Tallies: the per-cell energy tally lives in LDS (nc <= 4096 doubles) per block and is
-// flushed with FP64 atomics at the end; consecutive events of one particle in the same
-// (cell, group) are merged in registers before the FP64 atomic into grd_jrad.
+// Tallies: the per-cell energy tally lives in LDS (nc doubles) per block and is flushed
+// with FP64 atomics at the end; consecutive events of one particle in the same
+// (cell, group) are merged in registers before the FP64 atomic into one of the grd_jrad
+// replicas (block-indexed), which k_jrad_reduce folds together afterwards.
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
+#include "slgpu_sample.h"
 
 namespace slgpu {
 
@@ -32,34 +35,6 @@ namespace slgpu {
 #define WLI(ig) (d.wlinv[(ig) - 1])
 #define DIVV(ix) (d.divv[(ix) - 1])
 #define TINV(ic) (d.tempinv[(ic) - 1])
-
-// first index j in [lo, hi) with cum[j] > r (upper bound); hi if none
-__device__ __forceinline__ int first_above(const double *__restrict__ cum, int lo, int hi, double r)
-{
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (cum[mid] > r) hi = mid;
-        else lo = mid + 1;
-    }
-    return lo;
-}
-
-// GRID/emitgroup.f:3-54 via the per-cell in-bin cumulative table
-__device__ __forceinline__ int emitgroup(const Dev &d, double r, int ic)
-{
-    const int ng = d.ng;
-    const double *ep = d.emitprob + (size_t)(ic - 1) * d.nep;
-    const double r1 = r * d.capgrey[ic - 1];
-    int iep = binsrch(r1, ep, d.nep, true);
-    if (iep > d.nep - 1) iep = d.nep - 1;
-    if (iep < 0) iep = 0;
-    const int ig = iep * d.nepg + 1;
-    const int igp1 = ig + d.nepg - 1 < ng ? ig + d.nepg - 1 : ng;
-    // first group g in [ig, igp1-1] whose running sum exceeds r1, else igp1
-    const double *cum = d.emitcum + (size_t)(ic - 1) * ng;
-    int j = first_above(cum, ig - 1, igp1 - 1, r1);
-    return j + 1;
-}
 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sources(Dev d, uint32_t iteration)
@@ -234,8 +209,10 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         e0 = e0 * help;
         if (ng != 1) {
             r1 = s.draw();
-            ig = emitgroup(d, r1, ic);
-            if (((SIGT(ic) + CAPT(ig, ic)) * DXX(ix) >= d.tauddmc) && !d.puretran) p.itype = 2;
+            int thin;
+            ig = sample_group(d, TAB_E, ic, 0, r1, thin);
+            if (thin < 0) thin = is_thin(d, ig, ic);
+            if (!thin) p.itype = 2;
         }
         if (p.itype == 2) {
             evelo = evelo + e * vx * mu * CINV;
@@ -381,7 +358,59 @@ __device__ __forceinline__ double shell_position(const Dev &d, int ix, double r1
     return x;
 }
 
-// TRANSPORT1/diffusion11.f90:91-631, every cell constant evaluated for the current cell
+// diffusion11.f90:243-283: the denominator and event thresholds of one DDMC step
+__device__ __forceinline__ DdmcProb ddmc_prob(double opl, double opr, double emitlump,
+                                              double caplump, double doplump)
+{
+    double denom = opl + opr + (1.0 - emitlump) * caplump + doplump;
+    denom = 1.0 / denom;
+    const double pl = opl * denom, pr = opr * denom, pdop = doplump * denom;
+    DdmcProb q;
+    q.denom = denom;
+    q.t1 = 0.0 + pdop;
+    q.t2 = 0.0 + pdop + pl;
+    q.t3 = 0.0 + pdop + (pl + pr);
+    return q;
+}
+
+// The per-event constants of diffusion11.f90:95-152 and 204-239 depend only on the cell
+// and, for an unlumped group, on the group; they are evaluated once per iteration here,
+// one thread per (group, cell), and the step loads them.
+__global__ __launch_bounds__(256) void k_ddmc_probs(Dev d)
+{
+    const int ng = d.ng;
+    const int ig = blockIdx.x * 256 + threadIdx.x + 1, ic = blockIdx.y + 1, ix = ic;
+    if (ig > ng) return;
+    const CellDDMC cc = d.cell[ic - 1];
+    const double cp = CAPT(ig, ic);
+    const double emitlump = specint0(TINV(ic), d.wlinv, ig) * cc.capgreyinv * cp;
+    const double sa = d.specarr[(size_t)(ic - 1) * ng + ig - 1];
+    double doplump;
+    if (ig != ng && DIVV(ix) >= 0)
+        doplump = dopspeccalc(TINV(ic), d.wlinv, ig) * DIVV(ix) / (sa * 3 * PC_C);
+    else if (ig != 1 && DIVV(ix) < 0)
+        doplump = -1 * dopspeccalc(TINV(ic), d.wlinv, ig - 1) * DIVV(ix) / (sa * 3 * PC_C);
+    else
+        doplump = 0.0;
+    const double opl = (ix == 1) ? 0.0 : resleak_t(d, ix, ig, -1);
+    const double opr = resleak_t(d, ix, ig, +1);
+    d.gprob[(size_t)(ic - 1) * ng + ig - 1] = ddmc_prob(opl, opr, emitlump, cp, doplump);
+    if (ig == 1) d.cprob[ic - 1] = ddmc_prob(OPL(1, ic), OPL(2, ic), cc.emitlump, cc.caplump, cc.doplump);
+}
+
+void launch_ddmc_probs(slgpu_ctx *c)
+{
+    hipLaunchKernelGGL(k_ddmc_probs, dim3((c->ng + 255) / 256, c->nx), dim3(256), 0, c->stream,
+                       c->d);
+}
+
+// TRANSPORT1/diffusion11.f90:91-631 — one DDMC event.  Every cell constant is evaluated for
+// the current cell.  The four events (Doppler, left leak, right leak, effective scatter,
+// :283-620) run through ONE code path: the event selects the sampling table, the
+// destination cell and the IMC-conversion rule by data, while the random draws keep the
+// reference's order per event.  That keeps the lanes of a wave on the same instructions.
+enum { EV_DOP = 0, EV_LEFT = 1, EV_RIGHT = 2, EV_SCAT = 3 };
+
 __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double &jrd,
                            double &evelo)
 {
@@ -389,124 +418,54 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     int ix = p.ix, ig = p.ig;
     const int ic = ix;
     double x = p.x, mu = p.mu, e = p.e, e0 = p.e0, wl = p.wl, vx = p.vx;
-    double r1, r2;
-    int iiig = ig;
+    double r1;
     erd = 0.0; jrd = 0.0;
     const double dist = DXX(ix);
-    const CellDDMC cc = d.cell[ic - 1];
+    const int nlump = d.cell[ic - 1].nlump;
     const double cp = CAPT(ig, ic);
-    const int glump = (cp * dist >= d.taulump) ? cc.nlump : 0;
+    const int glump = (cp * dist >= d.taulump) ? nlump : 0;
     if ((SIGT(ic) + cp) * dist < d.tauddmc) return 100;
-    double emitlump, caplump, doplump;
-    double opl, opr;
-    if (glump > 0) {
-        emitlump = cc.emitlump;
-        caplump = cc.caplump;
-        doplump = cc.doplump;
-        opl = OPL(1, ic);
-        opr = OPL(2, ic);
-    } else {
-        emitlump = specint0(TINV(ic), d.wlinv, ig) * cc.capgreyinv * cp;
-        caplump = cp;
-        const double sa = d.specarr[(size_t)(ic - 1) * ng + ig - 1];
-        if (ig != ng && DIVV(ix) >= 0)
-            doplump = dopspeccalc(TINV(ic), d.wlinv, ig) * DIVV(ix) / (sa * 3 * PC_C);
-        else if (ig != 1 && DIVV(ix) < 0)
-            doplump = -1 * dopspeccalc(TINV(ic), d.wlinv, ig - 1) * DIVV(ix) / (sa * 3 * PC_C);
-        else
-            doplump = 0.0;
-        opl = (ix == 1) ? 0.0 : resleak_t(d, ix, ig, -1);
-        opr = resleak_t(d, ix, ig, +1);
-    }
-    double denom = opl + opr + (1.0 - emitlump) * caplump + doplump;
-    denom = 1.0 / denom;
+    // one load path for both kinds of group: the cell's lumped thresholds or the group's own
+    const DdmcProb *qp = glump > 0 ? d.cprob + (ic - 1) : d.gprob + (size_t)(ic - 1) * ng + (ig - 1);
+    const DdmcProb q = *qp;
+    const double denom = q.denom;
     r1 = s.draw();
     const double ddmct = fabs(log(r1) * denom * CINV);
     erd = e * ddmct;
     jrd = PC_C * erd;
     r1 = s.draw();
-    const double pl = opl * denom, pr = opr * denom;
-    const double pdop = doplump * denom;
+    int ev;
+    if (r1 >= 0.0 && r1 < q.t1) ev = EV_DOP;
+    else if (r1 >= q.t1 && r1 < q.t2) ev = EV_LEFT;
+    else if (r1 >= q.t2 && r1 < q.t3) ev = EV_RIGHT;
+    else ev = EV_SCAT;
+    p.idist = ev == EV_LEFT ? -3 : (ev == EV_RIGHT ? -4 : (ev == EV_SCAT ? -2 : p.idist));
     const size_t row = (size_t)(ic - 1) * ng;
     int ret = 0;
-
-    if (r1 >= 0.0 && r1 < 0.0 + pdop) {
-        // Doppler shift out of the group / lump
-        if (glump == 0) iiig = ig;
-        else {
-            r1 = s.draw();
-            const int n = cc.nlump;
-            int j = guided_first_above(d.cumD + row, n, d.guide + ((size_t)2 * nx + ic - 1) * GUIDE_K, d.guide_m[2 * nx + ic - 1], r1);
-            if (DIVV(ix) >= 0) iiig = d.gorder[row + (j < n ? j : n - 1)];
-            else iiig = d.gorder[row + (j < n ? n - 1 - j : 0)];
-        }
-        if (DIVV(ix) >= 0) {
-            ig = iiig + 1;
-            wl = WLG(ig);
-            if (ig > ng) ig = ng;
-        } else {
-            ig = iiig - 1;
-            wl = WLG(ig + 1);
-            if (ig < 1) ig = 1;
-        }
-        if ((SIGT(ic) + CAPT(ig, ic)) * dist < d.tauddmc) {
-            p.itype = 1;
-            r1 = s.draw();
-            mu = 1.0 - 2.0 * r1;
-            r1 = s.draw();
-            x = shell_position(d, ix, r1);
-            vx = (VXA(ix) * (XA(ix + 1) - x) + VXA(ix + 1) * (x - XA(ix))) / DXX(ix);
-            mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
-            wl = wl * (1.0 - vx * mu * CINV);
-            const double help = 1.0 / (1.0 - vx * mu * CINV);
-            evelo = evelo + e * (1.0 - help);
-            e = e * help;
-            e0 = e0 * help;
-        }
-    } else if (r1 >= 0.0 + pdop && r1 < 0.0 + pdop + pl) {
-        // left leakage
-        p.idist = -3;
-        if (ix == 1) goto ret_early;
-        const int l = ix - 1;
-        if (glump == 0) iiig = ig;
-        else {
-            r1 = s.draw();
-            const int n = cc.nlump;
-            int j = guided_first_above(d.cumL + row, n, d.guide + ((size_t)0 * nx + ic - 1) * GUIDE_K, d.guide_m[0 * nx + ic - 1], r1);
-            iiig = d.gorder[row + (j < n ? j : n - 1)];
-        }
-        if ((SIGT(l) + CAPT(iiig, l)) * DXX(ix - 1) < d.tauddmc) {
-            p.itype = 1;
-            r1 = s.draw();
-            wl = 1.0 / (r1 * WLI(iiig + 1) + (1.0 - r1) * WLI(iiig));
-            x = XA(ix);
-            vx = VXA(ix);
-            r1 = s.draw();
-            r2 = s.draw();
-            mu = -(r1 > r2 ? r1 : r2);
-            mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
-            const double help = 1.0 / (1.0 - vx * mu * CINV);
-            evelo = evelo + e * (1.0 - help);
-            e = e * help;
-            e0 = e0 * help;
-            wl = wl * (1.0 - vx * mu * CINV);
-        }
-        ix = ix - 1;
-        ig = iiig;
-    } else if (r1 >= 0.0 + pdop + pl && r1 < 0.0 + pdop + (pl + pr)) {
-        // right leakage
-        p.idist = -4;
-        if (ix == nx) {
+    bool adiabatic = true;
+    if (ev == EV_LEFT && ix == 1) {
+        adiabatic = false;                                     // :362-365 returns
+    } else if (ev == EV_SCAT && glump == ng) {
+        ret = 102;                                             // :552-556
+        adiabatic = false;
+    } else {
+        const bool flux = (ev == EV_RIGHT && ix == nx);        // :438-478
+        if (flux) {
             p.stat = 1;
+            const double ra = s.draw(), rb = s.draw();
+            mu = ra > rb ? ra : rb;
+        }
+        // group sampling: lumped particles draw from the cell's table of this event,
+        // unlumped effective scatters from the emission probabilities (emitgroup)
+        int iiig = ig, thin = -1;
+        if (glump > 0 || ev == EV_SCAT) {
             r1 = s.draw();
-            r2 = s.draw();
-            mu = r1 > r2 ? r1 : r2;
-            if (glump != 0) {
-                r1 = s.draw();
-                const int n = cc.nlump;
-                int j = guided_first_above(d.cumR + row, n, d.guide + ((size_t)1 * nx + ic - 1) * GUIDE_K, d.guide_m[1 * nx + ic - 1], r1);
-                ig = d.gorder[row + (j < n ? j : n - 1)];
-            }
+            const int t = glump == 0 ? TAB_E
+                        : (ev == EV_LEFT ? TAB_L : (ev == EV_RIGHT ? TAB_R : (ev == EV_DOP ? TAB_D : TAB_S)));
+            iiig = sample_group(d, t, ic, nlump, r1, thin);
+        }
+        if (flux) {
+            ig = iiig;
             r1 = s.draw();
             wl = 1.0 / (r1 * WLI(ig + 1) + (1.0 - r1) * WLI(ig));
             x = XA(nx + 1);
@@ -518,99 +477,118 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             wl = wl / help;
             e = e * help;
             e0 = e0 * help;
-            goto ret_early;
-        }
-        const int l = ix + 1;
-        if (glump == 0) iiig = ig;
-        else {
-            r1 = s.draw();
-            const int n = cc.nlump;
-            int j = guided_first_above(d.cumR + row, n, d.guide + ((size_t)1 * nx + ic - 1) * GUIDE_K, d.guide_m[1 * nx + ic - 1], r1);
-            iiig = d.gorder[row + (j < n ? j : n - 1)];
-        }
-        if ((SIGT(l) + CAPT(iiig, l)) * DXX(ix + 1) < d.tauddmc) {
-            p.itype = 1;
-            r1 = s.draw();
-            wl = 1.0 / (r1 * WLI(iiig + 1) + (1.0 - r1) * WLI(iiig));
-            x = XA(ix + 1);
-            vx = VXA(ix + 1);
-            r1 = s.draw();
-            r2 = s.draw();
-            mu = r1 > r2 ? r1 : r2;
-            mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
-            const double help = 1.0 / (1.0 - vx * mu * CINV);
-            evelo = evelo + e * (1.0 - help);
-            e = e * help;
-            e0 = e0 * help;
-            wl = wl * (1.0 - vx * mu * CINV);
-        }
-        ix = ix + 1;
-        ig = iiig;
-    } else {
-        // effective scattering
-        p.idist = -2;
-        if (glump == ng) { ret = 102; goto ret_early; }
-        if (glump == 0) {
-            r1 = s.draw();
-            iiig = emitgroup(d, r1, ic);
+            adiabatic = false;
         } else {
-            r1 = s.draw();
-            const int n = cc.nlump;
-            int j = n + guided_first_above(d.cumS + row + n, ng - n, d.guide + ((size_t)3 * nx + ic - 1) * GUIDE_K, d.guide_m[3 * nx + ic - 1], r1);
-            iiig = d.gorder[row + (j < ng ? j : ng - 1)];
+            // destination group and the cell whose optical depth decides IMC conversion
+            int ig2 = iiig, lc = ic;
+            if (ev == EV_DOP) {
+                if (DIVV(ix) >= 0) { ig2 = iiig + 1; wl = WLG(ig2); ig2 = ig2 > ng ? ng : ig2; }
+                else { ig2 = iiig - 1; wl = WLG(ig2 + 1); ig2 = ig2 < 1 ? 1 : ig2; }
+            } else if (ev == EV_LEFT) {
+                lc = ix - 1;
+            } else if (ev == EV_RIGHT) {
+                lc = ix + 1;
+            }
+            if (thin < 0) thin = is_thin(d, ig2, lc);
+            if (thin) {
+                // conversion to IMC (:329-355, 402-423, 515-538, 587-618)
+                p.itype = 1;
+                const bool face = (ev == EV_LEFT || ev == EV_RIGHT);
+                if (ev == EV_SCAT) {
+                    r1 = s.draw();
+                    wl = 1.0 / ((1.0 - r1) * WLI(ig2) + r1 * WLI(ig2 + 1));
+                } else if (face) {
+                    r1 = s.draw();
+                    wl = 1.0 / (r1 * WLI(iiig + 1) + (1.0 - r1) * WLI(iiig));
+                }
+                const double ra = s.draw(), rb = s.draw();
+                if (face) {
+                    const int xi = ev == EV_LEFT ? ix : ix + 1;
+                    x = XA(xi);
+                    vx = VXA(xi);
+                    mu = ra > rb ? ra : rb;
+                    if (ev == EV_LEFT) mu = -mu;
+                } else {
+                    mu = 1.0 - 2.0 * ra;
+                    x = shell_position(d, ix, rb);
+                    vx = (VXA(ix) * (XA(ix + 1) - x) + VXA(ix + 1) * (x - XA(ix))) / DXX(ix);
+                }
+                mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
+                const double help = 1.0 / (1.0 - vx * mu * CINV);
+                evelo = evelo + e * (1.0 - help);
+                e = e * help;
+                e0 = e0 * help;
+                wl = wl * (1.0 - vx * mu * CINV);
+            }
+            ig = ig2;
+            if (ev == EV_LEFT) ix = ix - 1;
+            else if (ev == EV_RIGHT) ix = ix + 1;
         }
-        ig = iiig;
-        if ((SIGT(ic) + CAPT(ig, ic)) * dist < d.tauddmc) {
-            p.itype = 1;
-            r1 = s.draw();
-            wl = 1.0 / ((1.0 - r1) * WLI(ig) + r1 * WLI(ig + 1));
-            r1 = s.draw();
-            mu = 1.0 - 2.0 * r1;
-            r1 = s.draw();
-            x = shell_position(d, ix, r1);
-            vx = (VXA(ix) * (XA(ix + 1) - x) + VXA(ix + 1) * (x - XA(ix))) / DXX(ix);
-            mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
-            const double help = 1.0 / (1.0 - vx * mu * CINV);
-            evelo = evelo + e * (1.0 - help);
-            e = e * help;
-            e0 = e0 * help;
-            wl = wl * (1.0 - vx * mu * CINV);
+    }
+    if (adiabatic) {
+        if (x < XA(1)) {
+            p.stat = 2;                                        // :623-626
+        } else {
+            const double f = exp(-DIVV(ix) * ddmct / 3);
+            evelo = evelo + e * (1.0 - f);
+            e = e * f;
+            e0 = e0 * f;
         }
     }
-    if (x < XA(1)) {
-        p.stat = 2;
-        goto ret_early;
-    }
-    {
-        const double f = exp(-DIVV(ix) * ddmct / 3);
-        evelo = evelo + e * (1.0 - f);
-        e = e * f;
-        e0 = e0 * f;
-    }
-ret_early:
     p.x = x; p.mu = mu; p.e = e; p.e0 = e0; p.wl = wl; p.vx = vx;
     p.ix = ix; p.ig = ig;
     return ret;
 }
 
-// particle_advance.f90:140-294 as a persistent kernel; see the file header.
-#ifndef SLGPU_TRANSPORT_WAVES
-#define SLGPU_TRANSPORT_WAVES 2
+struct Accum {
+    double evelo, eout, edead, estart, esrc;
+    uint32_t nact, nimc, nddmc, nswap, nflux, ndead, nerr, nwarn, nmax;
+};
+
+__device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
+{
+    const double vals[5] = {a.eout, a.evelo, a.estart, a.edead, a.esrc};
+    const int slot[5] = {0, 1, 3, 4, 5};
+    for (int q = 0; q < 5; ++q) {
+        const double v = wave_sum(vals[q]);
+        if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&d.tot[slot[q]], v);
+    }
+    const uint32_t cv[8] = {a.nact, a.nimc, a.nddmc, a.nswap, a.nflux, a.ndead, a.nerr, a.nwarn};
+    const int cslot[8] = {0, 1, 2, 4, 5, 6, 7, 8};
+    for (int q = 0; q < 8; ++q) {
+        unsigned long long v = cv[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&d.ct[cslot[q]], v);
+    }
+    unsigned long long v = a.nmax;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(&d.ct[3], v);
+}
+
+// particle_advance.f90:140-294 as a persistent kernel: each lane takes the next particle
+// from a global counter, runs it event by event to escape or death (145-202 on start,
+// 271-292 at the end), then takes another, so lanes stay busy until the pool is empty.
+// Lanes that leave the loop drop out of the votes; a pass always runs the type of at
+// least one remaining lane, so every particle advances.
+// optional occupancy target of the transport kernel (waves per SIMD), a build-time knob
+#ifdef SLGPU_TRANSPORT_WAVES
+#define SLGPU_TRANSPORT_ATTR __attribute__((amdgpu_waves_per_eu(SLGPU_TRANSPORT_WAVES, 8)))
+#else
+#define SLGPU_TRANSPORT_ATTR
 #endif
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SLGPU_TRANSPORT_WAVES, 8)))
-void k_transport(Dev d, uint32_t iteration)
+__global__ __launch_bounds__(256) SLGPU_TRANSPORT_ATTR void k_transport(Dev d, uint32_t iteration)
 {
     extern __shared__ double s_tally[];
     const int ng = d.ng, nx = d.nx;
     for (int k = threadIdx.x; k < nx; k += blockDim.x) s_tally[k] = 0.0;
     __syncthreads();
-
-    double evelo = 0.0, eout = 0.0, edead = 0.0, estart = 0.0, esrc = 0.0;
-    unsigned long long nact = 0, nimc = 0, nddmc = 0, nswap = 0, nflux = 0, ndead = 0,
-                       nerr = 0, nwarn = 0, nmax = 0;
+    Accum a = {};
     int run_ic = -1, run_ig = -1;
     double run_e = 0.0, run_j = 0.0;
-
+    double *jrad = d.jrad + (size_t)(blockIdx.x % d.jrad_copies) * nx * ng;
     Ptcl p;
     Philox s;
     int64_t ip = -1, istep = 0;
@@ -619,15 +597,17 @@ void k_transport(Dev d, uint32_t iteration)
         if (!have) {
             ip = (int64_t)atomicAdd(d.queue, 1ull);
             if (ip >= d.npart) break;
-            // particle_advance.f90:145-202
             p.x = d.px[ip]; p.mu = d.pmu[ip]; p.e = d.pe[ip]; p.e0 = d.pe0[ip]; p.wl = d.pwl[ip];
-            esrc += p.e;
+            a.esrc += p.e;
+            a.nact++;
             p.ix = binsrch(p.x, d.xarr, nx + 1, false);
             p.ig = binsrch(p.wl, d.wl, ng + 1, false);
-            bool bad = p.ix < 1 || p.ig < 1;     // NaN position or wavelength
-            if (bad) { p.ix = 1; p.ig = 1; }
-            const double help = DXX(p.ix);
-            const double tau = (SIGT(p.ix) + CAPT(p.ig, p.ix)) * help;
+            if (p.ix < 1 || p.ig < 1) {                        // NaN position or wavelength
+                a.nerr++; a.ndead++; a.edead += p.e;
+                d.pstat[ip] = 2; d.perr[ip] = 1; d.pnstep[ip] = 0;
+                continue;
+            }
+            const double tau = (SIGT(p.ix) + CAPT(p.ig, p.ix)) * DXX(p.ix);
             p.itype = (d.puretran || tau < d.tauddmc) ? 1 : 2;
             const double h1 = XA(p.ix), h2 = XA(p.ix + 1);
             p.vx = (VXA(p.ix) * (h2 - p.x) + VXA(p.ix + 1) * (p.x - h1)) / (h2 - h1);
@@ -637,71 +617,55 @@ void k_transport(Dev d, uint32_t iteration)
                 p.e = p.e / labfact;
                 p.e0 = p.e0 / labfact;
             }
-            estart += p.e;
+            a.estart += p.e;
             p.stat = 0;
             p.idist = 0;
             s.init(d.seed, iteration, (uint32_t)ip, 1u);
             istep = 0;
-            nact++;
             have = true;
-            if (bad) {
-                nerr++;
-                ndead++;
-                edead += p.e;
-                d.pstat[ip] = 2;
-                d.perr[ip] = 1;
-                d.pnstep[ip] = 0;
-                have = false;
-                continue;
-            }
         }
-        // one event
+        // One event.  The wave runs one event type per pass, the one most of its lanes
+        // hold; lanes of the other type wait a pass, so each pass executes one body.
+        const bool imc = p.itype == 1 || d.puretran;
+        const int n_imc = __popcll(__ballot(imc)), n_ddmc = __popcll(__ballot(!imc));
+        if (imc != (n_imc >= n_ddmc)) continue;
         istep++;
         const int icold = p.ix, igold = p.ig;
         double erd, jrd;
         int ierr;
-        if (p.itype == 1 || d.puretran) {
-            nimc++;
-            ierr = transport11(d, p, s, erd, jrd, evelo);
-            if (p.itype != 1) nswap++;
+        if (imc) {
+            a.nimc++;
+            ierr = transport11(d, p, s, erd, jrd, a.evelo);
+            if (p.itype != 1) a.nswap++;
         } else {
-            nddmc++;
-            ierr = diffusion11(d, p, s, erd, jrd, evelo);
-            if (p.itype == 1) nswap++;
+            a.nddmc++;
+            ierr = diffusion11(d, p, s, erd, jrd, a.evelo);
+            if (p.itype == 1) a.nswap++;
         }
         if (icold != run_ic || igold != run_ig) {
             if (run_ic > 0) {
                 atomicAdd(&s_tally[run_ic - 1], run_e);
-                atomicAdd(&d.jrad[(size_t)(run_ic - 1) * ng + run_ig - 1], run_j);
+                atomicAdd(&jrad[(size_t)(run_ic - 1) * ng + run_ig - 1], run_j);
             }
             run_ic = icold; run_ig = igold; run_e = 0.0; run_j = 0.0;
         }
         run_e += erd;
         run_j += jrd;
-        if (p.itype == 1 && p.stat == 0) {
+        if (p.itype == 1 && p.stat == 0) {                     // particle_advance.f90:234-250
             if (p.x > XA(p.ix + 1) || p.x < XA(p.ix) || p.x != p.x) {
                 if (ierr == 0) ierr = -99;
             }
         }
-        if (ierr < 0) nwarn++;
+        if (ierr < 0) a.nwarn++;
         if (ierr == 0 && p.stat == 0 && istep >= d.max_steps) ierr = 200;
-        if (ierr > 0) {
-            nerr++;
-            p.stat = 2;
-        }
+        if (ierr > 0) { a.nerr++; p.stat = 2; }
         if (p.stat != 0) {
-            // particle_advance.f90:271-292
-            if ((unsigned long long)istep > nmax) nmax = istep;
-            if (p.stat == 2) {
-                ndead++;
-                edead += p.e;
-            } else {
-                nflux++;
-                eout += p.e;
-            }
+            if ((uint32_t)istep > a.nmax) a.nmax = (uint32_t)istep;
+            if (p.stat == 2) { a.ndead++; a.edead += p.e; }
+            else { a.nflux++; a.eout += p.e; }
             d.pstat[ip] = (uint8_t)p.stat;
             d.px[ip] = p.x; d.pmu[ip] = p.mu; d.pwl[ip] = p.wl; d.pe[ip] = p.e; d.pe0[ip] = p.e0;
-            d.pig[ip] = p.ig;
+            d.pig[ip] = p.ig; d.pix[ip] = p.ix;
             d.pnstep[ip] = istep;
             d.perr[ip] = ierr > 0 ? ierr : 0;
             have = false;
@@ -709,34 +673,12 @@ void k_transport(Dev d, uint32_t iteration)
     }
     if (run_ic > 0) {
         atomicAdd(&s_tally[run_ic - 1], run_e);
-        atomicAdd(&d.jrad[(size_t)(run_ic - 1) * ng + run_ig - 1], run_j);
+        atomicAdd(&jrad[(size_t)(run_ic - 1) * ng + run_ig - 1], run_j);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nx; k += blockDim.x)
         if (s_tally[k] != 0.0) atomicAdd(&d.tally[k], s_tally[k]);
-
-    // block reductions of totals and counters
-    const double vals[5] = {eout, evelo, estart, edead, esrc};
-    const int slot[5] = {0, 1, 3, 4, 5};
-    for (int q = 0; q < 5; ++q) {
-        double v = wave_sum(vals[q]);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&d.tot[slot[q]], v);
-    }
-    const unsigned long long cv[8] = {nact, nimc, nddmc, nswap, nflux, ndead, nerr, nwarn};
-    const int cslot[8] = {0, 1, 2, 4, 5, 6, 7, 8};
-    for (int q = 0; q < 8; ++q) {
-        unsigned long long v = cv[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&d.ct[cslot[q]], v);
-    }
-    {
-        unsigned long long v = nmax;
-        for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(v, off, 64);
-            v = o > v ? o : v;
-        }
-        if ((threadIdx.x & 63) == 0) atomicMax(&d.ct[3], v);
-    }
+    flush_accum(d, a);
 }
 
 // fluxtally.f:30-54 (flux grid == group grid; polar/azimuth bins are single)
@@ -767,10 +709,59 @@ void launch_transport(slgpu_ctx *c, uint32_t iteration)
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     int64_t blocks = (int64_t)ncu * 8;
-    int64_t need = (c->d.npart + 255) / 256;
+    const int64_t need = (c->d.npart + 255) / 256;
     if (need < blocks) blocks = need > 0 ? need : 1;
     hipLaunchKernelGGL(k_transport, dim3((unsigned)blocks), dim3(256), c->nx * sizeof(double),
                        c->stream, c->d, iteration);
+}
+
+// the transport's group samplers at arbitrary (r, cell), table path or direct search
+__global__ __launch_bounds__(256) void k_sample_groups(Dev d, int t, int direct, int64_t n,
+                                                       const double *r, const int *cell, int *out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int ic = cell[k];
+    if (ic < 1 || ic > d.nx) { out[k] = -1; return; }
+    const int nlump = d.cell[ic - 1].nlump;
+    if (t != TAB_E) {
+        const LumpRow L = lump_row(d, t, ic, nlump);
+        if (L.cnt == 0) { out[k] = 0; return; }
+        if (direct) { out[k] = lump_group(d, L, ic, first_above(L.cum, 0, L.cnt, r[k])); return; }
+    } else if (direct) {
+        out[k] = emitgroup_direct(d, r[k], ic);
+        return;
+    }
+    int thin;
+    out[k] = sample_group(d, t, ic, nlump, r[k], thin);
+}
+
+void launch_sample_groups(slgpu_ctx *c, int t, int direct, int64_t n, const double *r,
+                          const int *cell, int *out)
+{
+    hipLaunchKernelGGL(k_sample_groups, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, c->d, t, direct, n, r, cell, out);
+}
+
+// fold the grd_jrad replicas into replica 0 (fixed order per element)
+__global__ __launch_bounds__(256) void k_jrad_reduce(Dev d)
+{
+    const size_t n = (size_t)d.nx * d.ng;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        double s = d.jrad[k];
+        for (int r = 1; r < d.jrad_copies; ++r) s += d.jrad[(size_t)r * n + k];
+        d.jrad[k] = s;
+    }
+}
+
+void launch_jrad_reduce(slgpu_ctx *c)
+{
+    if (c->d.jrad_copies <= 1) return;
+    const size_t n = (size_t)c->nx * c->ng;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_jrad_reduce, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->d);
 }
 
 void launch_fluxtally(slgpu_ctx *c)

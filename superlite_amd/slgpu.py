@@ -27,8 +27,8 @@ _u32 = ctypes.c_uint32
 EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_set_grid",
            "slgpu_set_groups", "slgpu_set_lines", "slgpu_set_bf_tables", "slgpu_set_ff_table",
            "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
-           "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_timers",
-           "slgpu_stream")
+           "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
+           "slgpu_timers", "slgpu_stream")
 
 
 class SlgpuError(RuntimeError):
@@ -73,6 +73,7 @@ def lib():
         L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp]
         L.slgpu_fluxtally.argtypes = [_vp, _vp, _vp, _vp]
         L.slgpu_get_particles.argtypes = [_vp, _i64] + [_vp] * 8
+        L.slgpu_sample_groups.argtypes = [_vp, _i, _i, _i64, _vp, _vp, _vp]
         L.slgpu_timers.argtypes = [_vp, _vp, _i, _i]
         L.slgpu_stream.restype = _vp
         L.slgpu_stream.argtypes = [_vp]
@@ -208,6 +209,15 @@ class Context:
         self._check(self.L.slgpu_get_particles(self.h, n, *[_p(pp[k]) for k in
                                                             ("stat", "x", "mu", "wl", "e", "ig", "nstep", "err")]))
         return pp
+
+    def sample_groups(self, table, r, cell, direct=False):
+        """group sampled by table (0-3 lumped L/R/D/S, 4 emitgroup) for draws r in cells"""
+        r = np.ascontiguousarray(r, np.float64)
+        cell = np.ascontiguousarray(cell, np.int32)
+        out = np.zeros(len(r), np.int32)
+        self._check(self.L.slgpu_sample_groups(self.h, int(table), int(direct), len(r), _p(r),
+                                               _p(cell), _p(out)))
+        return out
 
     def timers(self, reset=False):
         ms = np.zeros(8)

@@ -135,6 +135,41 @@ def test_ddmc_prep_c1(S, tables):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_sampler_tables_exact(S, tables, name):
+    """The bucket maps the transport samples groups with return exactly what the plain
+    searches return, for random draws and for draws on and next to every kind of bucket
+    edge; emitgroup also matches the oracle's emitgroup.f restatement."""
+    say(f"case: sampler tables {name}")
+    ctx, w, st, wl, lines = make_ctx(S, name, tables, npartmax=1024)
+    g = gpu_opacity(ctx, st)
+    p = ctx.ddmc_prep(st["temp"], w.lbol)
+    K = 1 << 14
+    rng = np.random.default_rng(11)
+    b = rng.integers(0, K, 2048).astype(np.float64)
+    edges = np.concatenate([b / K, np.nextafter((b + 1) / K, 0.0), np.nextafter(b / K, 1.0),
+                            np.nextafter(b / K, 0.0), [1.0, np.nextafter(1.0, 0.0), 2.0 ** -54]])
+    r = np.concatenate([rng.random(8192), edges])
+    r = r[(r > 0.0) & (r <= 1.0)]
+    cells = np.arange(1, w.nx + 1, max(1, w.nx // 24))
+    rr = np.tile(r, len(cells))
+    cc = np.repeat(cells, len(r)).astype(np.int32)
+    for t in range(5):
+        a = ctx.sample_groups(t, rr, cc)
+        ref = ctx.sample_groups(t, rr, cc, direct=True)
+        bad = np.flatnonzero(a != ref)
+        say(f"table {t}: {len(rr)} draws, {len(bad)} mismatches")
+        assert len(bad) == 0, (t, rr[bad[:5]], cc[bad[:5]], a[bad[:5]], ref[bad[:5]])
+        assert (a[(t == 4) | (a != 0)] >= 1).all() and (a <= w.ng).all()
+    m = oracle_model(st, w, wl, g, prep=p)
+    sel = rng.choice(len(rr), 3000, replace=False)
+    e = ctx.sample_groups(4, rr[sel], cc[sel])
+    o = np.array([m.emitgroup(rr[k], cc[k]) for k in sel])
+    say("emitgroup vs oracle agreement", (e == o).mean())
+    assert (e == o).mean() > 0.999
+    ctx.close()
+
+
 def test_sources_c1_per_particle(S, tables):
     say("case: sources c1")
     ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 14, seed=5)
