@@ -110,11 +110,13 @@ def main():
     K = args.steps
     value = steps / dt
     opac_rate = w.nx * w.ng * K / (tm["opacity"] * 1e-3)
-    # dominant kernel: transport; algorithmic bytes per particle-step = 4 (FP32 cap gather)
-    # + 2*8 (tally) + 2*8 (jrad) read-modify-write = 36 B, plus the particle state read and
-    # written once per particle (5 FP64 + 1 byte status + 3 int)
+    # dominant kernel: k_transport.  Algorithmic bytes per particle-step: the FP32 group
+    # opacity (4 B), the step's threshold record (32 B), one sampler-table entry (4 B) and
+    # the FP64 read-modify-write of the grd_jrad tally (16 B) = 56 B; per particle, its
+    # state read once and written once (5 FP64 in, 5 FP64 + status + group + cell + step
+    # count + error out = 40 + 40 + 1 + 4 + 4 + 8 + 4 B = 101 B)
     t_kernel = tm["transport_kernel"] * 1e-3 / K
-    bytes_kernel = 36.0 * steps / K + (2 * 5 * 8 + 1 + 12) * nstot
+    bytes_kernel = 56.0 * steps / K + 101.0 * nstot
     achieved = bytes_kernel / t_kernel / 1e9
     out = {
         "metric": "particle-steps/s + opacity cell·groups/s, FP32-mixed vs FP64 at ≤1% tally err",

@@ -567,8 +567,8 @@ int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double 
     hipEventRecord(c->ev[12], c->stream);
     HIPCHK(hipMemsetAsync(c->d.queue, 0, 8, c->stream));
     launch_transport(c, iteration);
-    launch_jrad_reduce(c);
     hipEventRecord(c->ev[13], c->stream);
+    launch_jrad_reduce(c);
     HIPCHK(hipGetLastError());
     double tot[8];
     unsigned long long ct[10];

@@ -115,6 +115,7 @@ __global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const doubl
 
 // Fused bb + bf + ff for one (cell, group); result into capT[g*nc + i] (cell-fastest).
 // blockDim = 256 = 64 cells x 4 groups; grid = (ceil(nc/64), ceil(ng/4)).
+template <bool M>
 __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
                                                   const double *__restrict__ rgg_tab)
 {
@@ -140,11 +141,27 @@ __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
         const double phi = wlc * wlc / (dwl * PC_C);
         const int l0 = d.gstart[g], l1 = d.gstart[g + 1];
         const double *ocg = d.ocg + ic;
-        for (int l = l0; l < l1; ++l) {
-            const LineRec r = d.lines[l];                 // uniform: scalar loads
-            const double o = ocg[(size_t)r.ion * nc];
-            const double term = phi * (double)r.gxs * o * exp(-((double)r.chilw * hckt)) * expfac;
-            acc += (o > 0.0) ? term : 0.0;
+        if constexpr (M) {
+            // precision form: FP32 terms, Kahan-compensated FP32 sum, widened once
+            const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
+            float sum = 0.0f, comp = 0.0f;
+            for (int l = l0; l < l1; ++l) {
+                const LineRec r = d.lines[l];
+                const double o = ocg[(size_t)r.ion * nc];
+                const float t = phi32 * r.gxs * (float)o * expf(-(r.chilw * hk32)) * ef32;
+                const float y = ((o > 0.0) ? t : 0.0f) - comp;
+                const float u = sum + y;
+                comp = (u - sum) - y;
+                sum = u;
+            }
+            acc += (double)sum;
+        } else {
+            for (int l = l0; l < l1; ++l) {
+                const LineRec r = d.lines[l];             // uniform: scalar loads
+                const double o = ocg[(size_t)r.ion * nc];
+                const double term = phi * (double)r.gxs * o * exp(-((double)r.chilw * hckt)) * expfac;
+                acc += (o > 0.0) ? term : 0.0;
+            }
         }
     }
     if (!(flags & SLGPU_NOBF)) {
@@ -280,8 +297,12 @@ void launch_opacity(slgpu_ctx *c, int flags)
                         (3.0 * PC_H * PC_ME * (PC_C * PC_C * PC_C * PC_C)) *
                         std::sqrt(PC_PI2 / (3 * PC_ME * PC_H * PC_C));
     dim3 grid((c->nx + 63) / 64, (c->ng + 3) / 4);
-    hipLaunchKernelGGL(k_opacity, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,
-                       (const double *)g_rgg_scratch(c));
+    if (c->d.precision == SLGPU_MIXED)
+        hipLaunchKernelGGL(k_opacity<true>, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,
+                           (const double *)g_rgg_scratch(c));
+    else
+        hipLaunchKernelGGL(k_opacity<false>, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,
+                           (const double *)g_rgg_scratch(c));
 }
 
 void launch_opacity_finish(slgpu_ctx *c)

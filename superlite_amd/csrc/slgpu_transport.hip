@@ -22,6 +22,12 @@
 #include "slgpu_internal.h"
 #include "slgpu_sample.h"
 
+// No fused multiply-add contraction in this file: the reference evaluates each product and
+// sum separately, and a contraction can turn an exact cancellation into a negative
+// rounding residue (the grazing shell-crossing root sqrt(x_l^2 - (1-mu^2)x^2) at x = x_l
+// becomes the square root of -ulp, a NaN path length).
+#pragma clang fp contract(off)
+
 namespace slgpu {
 
 #define CAPT(ig, ic) ((double)d.cap32[(size_t)((ic) - 1) * ng + ((ig) - 1)])
@@ -103,6 +109,34 @@ struct Ptcl {
 };
 
 // TRANSPORT1/transport11.f90:86-489 (trn_noampfact = .true.)
+// Arithmetic of the precision form.  SLGPU_MIXED evaluates the sampled path lengths and
+// event times (-ln r), the adiabatic factor and the r^3-uniform position in FP32; the
+// shell-crossing quadratic, the Newton-Raphson Doppler distance, the DDMC thresholds,
+// positions, energies and every tally stay FP64.
+template <bool M> __device__ __forceinline__ double p_log(double x)
+{
+    if constexpr (M) return (double)__logf((float)x);
+    else return log(x);
+}
+template <bool M> __device__ __forceinline__ double p_exp(double x)
+{
+    if constexpr (M) return (double)__expf((float)x);
+    else return exp(x);
+}
+// r^3-uniform radius in [xl, xr]: (r*xr^3 + (1-r)*xl^3)**(1.0/3.0) with the reference's
+// single-precision exponent (diffusion11.f90:336,597).  Radii cubed (~1e47 cm^3) overflow
+// FP32, so the precision form takes the root of the normalised r + (1-r)*(xl/xr)^3.
+template <bool M> __device__ __forceinline__ double p_shell_root(double r1, double xl, double xr)
+{
+    if constexpr (M) {
+        const double q = xl / xr;
+        return xr * (double)powf((float)(r1 + (1.0 - r1) * (q * q * q)), 1.0f / 3.0f);
+    } else {
+        return pow(r1 * (xr * xr * xr) + (1.0 - r1) * (xl * xl * xl), (double)(1.0f / 3.0f));
+    }
+}
+
+template <bool M>
 __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double &jrd,
                            double &evelo)
 {
@@ -126,10 +160,10 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     }
     if (db != db) return 1;
     const double sg = SIGT(ic), cp = CAPT(ig, ic);
-    if (sg > 0.0) { r1 = s.draw(); dthm = -log(r1) / (elabfact * sg); }
+    if (sg > 0.0) { r1 = s.draw(); dthm = -p_log<M>(r1) / (elabfact * sg); }
     else dthm = far;
     if (cp <= 0.0) dcol = far;
-    else { r1 = s.draw(); dcol = -log(r1) / (elabfact * cp); }
+    else { r1 = s.draw(); dcol = -p_log<M>(r1) / (elabfact * cp); }
     const double dvdr = (VXA(ix + 1) - VXA(ix)) / DXX(ix);
     const double vhelp = VXA(ix) - dvdr * xl;
     if (fabs(vx - vhelp) / PC_C >= 0.1) {
@@ -348,11 +382,11 @@ __device__ __forceinline__ double resleak_t(const Dev &d, int ix, int g, int sid
 }
 
 // uniform position in the shell with the single-precision 1/3 exponent, clamped (:336-339)
+template <bool M>
 __device__ __forceinline__ double shell_position(const Dev &d, int ix, double r1)
 {
-    const double third = (double)(1.0f / 3.0f);
     const double xl = XA(ix), xr = XA(ix + 1);
-    double x = pow(r1 * (xr * xr * xr) + (1.0 - r1) * (xl * xl * xl), third);
+    double x = p_shell_root<M>(r1, xl, xr);
     x = x > xr ? xr : x;
     x = x < xl ? xl : x;
     return x;
@@ -411,6 +445,7 @@ void launch_ddmc_probs(slgpu_ctx *c)
 // reference's order per event.  That keeps the lanes of a wave on the same instructions.
 enum { EV_DOP = 0, EV_LEFT = 1, EV_RIGHT = 2, EV_SCAT = 3 };
 
+template <bool M>
 __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double &jrd,
                            double &evelo)
 {
@@ -430,7 +465,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     const DdmcProb q = *qp;
     const double denom = q.denom;
     r1 = s.draw();
-    const double ddmct = fabs(log(r1) * denom * CINV);
+    const double ddmct = fabs(p_log<M>(r1) * denom * CINV);
     erd = e * ddmct;
     jrd = PC_C * erd;
     r1 = s.draw();
@@ -510,7 +545,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
                     if (ev == EV_LEFT) mu = -mu;
                 } else {
                     mu = 1.0 - 2.0 * ra;
-                    x = shell_position(d, ix, rb);
+                    x = shell_position<M>(d, ix, rb);
                     vx = (VXA(ix) * (XA(ix + 1) - x) + VXA(ix + 1) * (x - XA(ix))) / DXX(ix);
                 }
                 mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
@@ -529,7 +564,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         if (x < XA(1)) {
             p.stat = 2;                                        // :623-626
         } else {
-            const double f = exp(-DIVV(ix) * ddmct / 3);
+            const double f = p_exp<M>(-DIVV(ix) * ddmct / 3);
             evelo = evelo + e * (1.0 - f);
             e = e * f;
             e0 = e0 * f;
@@ -579,6 +614,7 @@ __device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
 #else
 #define SLGPU_TRANSPORT_ATTR
 #endif
+template <bool M>
 __global__ __launch_bounds__(256) SLGPU_TRANSPORT_ATTR void k_transport(Dev d, uint32_t iteration)
 {
     extern __shared__ double s_tally[];
@@ -635,11 +671,11 @@ __global__ __launch_bounds__(256) SLGPU_TRANSPORT_ATTR void k_transport(Dev d, u
         int ierr;
         if (imc) {
             a.nimc++;
-            ierr = transport11(d, p, s, erd, jrd, a.evelo);
+            ierr = transport11<M>(d, p, s, erd, jrd, a.evelo);
             if (p.itype != 1) a.nswap++;
         } else {
             a.nddmc++;
-            ierr = diffusion11(d, p, s, erd, jrd, a.evelo);
+            ierr = diffusion11<M>(d, p, s, erd, jrd, a.evelo);
             if (p.itype == 1) a.nswap++;
         }
         if (icold != run_ic || igold != run_ig) {
@@ -711,8 +747,12 @@ void launch_transport(slgpu_ctx *c, uint32_t iteration)
     int64_t blocks = (int64_t)ncu * 8;
     const int64_t need = (c->d.npart + 255) / 256;
     if (need < blocks) blocks = need > 0 ? need : 1;
-    hipLaunchKernelGGL(k_transport, dim3((unsigned)blocks), dim3(256), c->nx * sizeof(double),
-                       c->stream, c->d, iteration);
+    if (c->d.precision == SLGPU_MIXED)
+        hipLaunchKernelGGL(k_transport<true>, dim3((unsigned)blocks), dim3(256),
+                           c->nx * sizeof(double), c->stream, c->d, iteration);
+    else
+        hipLaunchKernelGGL(k_transport<false>, dim3((unsigned)blocks), dim3(256),
+                           c->nx * sizeof(double), c->stream, c->d, iteration);
 }
 
 // the transport's group samplers at arbitrary (r, cell), table path or direct search
