@@ -170,6 +170,42 @@ def test_sampler_tables_exact(S, tables, name):
     ctx.close()
 
 
+def test_precision_form_c1(S, tables):
+    """FP32-mixed against FP64 on the same inputs and seeds: opacities within 1e-5, the
+    per-shell deposition and the escaped energy within 1%, the spectrum difference far
+    inside the Monte Carlo noise, and energy still conserved to 1e-12."""
+    say("case: precision form c1")
+    n2s = 20
+    out = {}
+    for prec in (S.SLGPU_FP64, S.SLGPU_MIXED):
+        ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << n2s, seed=9,
+                                         precision=prec)
+        op = gpu_opacity(ctx, st)
+        ctx.ddmc_prep(st["temp"], w.lbol, want=False)
+        ctx.sources(1 << n2s, 1)
+        tr = ctx.transport(1)
+        fl = ctx.fluxtally()
+        out[prec] = (op, tr, fl)
+        ctx.close()
+    (o0, t0, f0), (o1, t1, f1) = out[S.SLGPU_FP64], out[S.SLGPU_MIXED]
+    m = o0["cap64"] > 0
+    rel = np.abs(o1["cap64"][m] - o0["cap64"][m]) / o0["cap64"][m]
+    say("cap64 max rel", rel.max())
+    assert rel.max() < 1e-5
+    for k in ("capgrey", "capross"):
+        assert np.max(np.abs(o1[k] - o0[k]) / o0[k]) < 1e-5, k
+    tally = np.abs(t1["tally"] - t0["tally"]) / np.abs(t0["tally"])
+    say("shell tally max rel", tally.max(), "eout rel", abs(t1["eout"] - t0["eout"]) / t0["eout"])
+    assert tally.max() < 1e-2
+    assert abs(t1["eout"] - t0["eout"]) / t0["eout"] < 1e-2
+    sel = f0["lumnum"] > 100
+    z = (f1["luminos"][sel] - f0["luminos"][sel]) / np.sqrt(f0["lumdev"][sel] + f1["lumdev"][sel])
+    say("spectrum z rms", np.sqrt(np.mean(z ** 2)))
+    assert np.sqrt(np.mean(z ** 2)) < 0.5
+    for t in (t0, t1):
+        assert abs(t["estart"] - (t["eout"] + t["evelo"] + t["edead"])) < 1e-12 * t["estart"]
+
+
 def test_sources_c1_per_particle(S, tables):
     say("case: sources c1")
     ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 14, seed=5)
