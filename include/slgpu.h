@@ -136,6 +136,21 @@ int slgpu_fluxtally(slgpu_ctx *ctx, double *out_luminos, double *out_lumdev,
 int slgpu_get_particles(slgpu_ctx *ctx, int64_t n, uint8_t *stat, double *x, double *mu,
                         double *wl, double *e, int *ig, int64_t *nstep, int *err);
 
+/* Differentiation form: vector-Jacobian product of the last slgpu_opacity_build.
+ * Backward of physical_opacity.f:93-247 (bb, bf, ff), opacity_planckmean.f:18-24 and
+ * opacity_rossmean.f:18-21 with respect to the build's own inputs.  grad_cap64 (ng*nc, the
+ * layout of out_cap64), grad_capgrey (nc) and grad_capross (nc) are dL/d(output); any may be
+ * NULL (zero).  Outputs, any may be NULL: dL/dT (nc), dL/dnatom (nc), dL/dn_e (nc),
+ * dL/dnatomfr (30*nc, the layout of slgpu_gas.natomfr; 0 for elements absent from every
+ * cell) and dL/doc (nion*nc, the layout of slgpu_gas.oc).  The means go through the
+ * FP32-rounded cap as in the forward, the rounding taken as the identity.  Density and
+ * composition enter the opacity through the EOS populations (eos_update.f:37-45), which
+ * the host computes: dL/drho and dL/dX follow by chaining these outputs through it.  Line
+ * terms are recomputed, not stored.  Needs nion + (elements present) <= 128. */
+int slgpu_opacity_vjp(slgpu_ctx *ctx, const double *grad_cap64, const double *grad_capgrey,
+                      const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
+                      double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc);
+
 /* Verification hook for the transport's group samplers; no single reference routine
  * corresponds.  It evaluates, for n (draw r, cell) pairs, the sampler the transport
  * uses: table 0-3 = the lumped loops of diffusion11.f90 (left leak :375-398, right leak
@@ -148,7 +163,8 @@ int slgpu_sample_groups(slgpu_ctx *ctx, int table, int direct, int64_t n, const 
 
 /* accumulated device time per phase in ms, measured with HIP events on the context
  * stream: [0] opacity, [1] ddmc prep, [2] sources, [3] transport, [4] fluxtally,
- * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone; reset = 1 zeroes them */
+ * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone, [7] opacity backward;
+ * reset = 1 zeroes them */
 int slgpu_timers(slgpu_ctx *ctx, double *ms, int n, int reset);
 
 /* the HIP stream all work of the context is ordered on (hipStream_t) */

@@ -185,6 +185,12 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.radtemp, nc);
     rc |= dalloc(c, &d.srccum, cg);
     rc |= dalloc(c, &d.cell, nc);
+    rc |= dalloc(c, &d.natom_in, nc);
+    rc |= dalloc(c, &d.vol_in, nc);
+    rc |= dalloc(c, &d.nelec_in, nc);
+    rc |= dalloc(c, &d.gcapT, cg);
+    rc |= dalloc(c, &d.gcell, 4 * nc);
+    rc |= dalloc(c, &d.gfr, 30 * nc);
     rc |= dalloc(c, &d.cprob, nc);
     rc |= dalloc(c, &d.gprob, cg);
     rc |= dalloc(c, &d.gorder, cg);
@@ -305,10 +311,14 @@ int slgpu_set_ions(slgpu_ctx *c, int nion, const int *ion_iz, const int *ion_ii)
     c->h_ion_ii.assign(ion_ii, ion_ii + nion);
     dfree(c, c->d.ocg);
     dfree(c, c->d.ocT);
+    dfree(c, c->d.ginvT);
+    dfree(c, c->d.goc);
     dfree(c, (void *)c->d.bfxs);
     const size_t nc = c->nx;
     int rc = 0;
     rc |= dalloc(c, &c->d.ocg, (size_t)std::max(nion, 1) * nc);
+    rc |= dalloc(c, &c->d.ginvT, (size_t)std::max(nion, 1) * nc);
+    rc |= dalloc(c, &c->d.goc, (size_t)std::max(nion, 1) * nc);
     rc |= dalloc(c, &c->d.ocT, (size_t)(nion + 30) * nc);     // tail: ff coordinates per element
     rc |= dalloc(c, (double **)&c->d.bfxs, (size_t)std::max(nion, 1) * c->ng);
     if (rc) return rc;
@@ -443,11 +453,9 @@ int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *o
     c->d.nel = nel;
     PhaseTimer tm(c, 0);
     // per-call host inputs (small: nc and nion*nc doubles)
-    double *dn = nullptr, *dv = nullptr, *de = nullptr, *doc = nullptr, *dgi = nullptr;
+    double *dn = c->d.natom_in, *dv = c->d.vol_in, *de = c->d.nelec_in;   // kept for the backward
+    double *doc = nullptr, *dgi = nullptr;
     int rc = 0;
-    rc |= dalloc(c, &dn, nc);
-    rc |= dalloc(c, &dv, nc);
-    rc |= dalloc(c, &de, nc);
     rc |= dalloc(c, &doc, (size_t)std::max(nion, 1) * nc);
     rc |= dalloc(c, &dgi, (size_t)std::max(nion, 1) * nc);
     if (rc) return rc;
@@ -479,7 +487,8 @@ int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *o
     float kms = 0.f;
     hipEventElapsedTime(&kms, c->ev[10], c->ev[11]);
     c->t_ms[5] += kms;
-    dfree(c, dn); dfree(c, dv); dfree(c, de); dfree(c, doc); dfree(c, dgi);
+    dfree(c, doc); dfree(c, dgi);
+    c->opac_flags = flags;
     if (rc) return rc;
     c->have_opacity = true;
     c->have_prep = false;
@@ -626,6 +635,42 @@ int slgpu_get_particles(slgpu_ctx *c, int64_t n, uint8_t *stat, double *x, doubl
     if (err) HIPCHK(hipMemcpyAsync(err, c->d.perr, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad_capgrey,
+                      const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
+                      double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_opacity) return fail(SLGPU_EORDER, "slgpu_opacity_build must run first");
+    const int nc = c->nx, ng = c->ng, nion = c->nion;
+    if ((size_t)(nion + c->d.nel) * 64 * sizeof(double) > 65536)
+        return fail(SLGPU_EINVAL, "opacity backward: ions + elements exceed 128 per cell");
+    PhaseTimer tm(c, 7);
+    const size_t cg = (size_t)nc * ng;
+    double *dgc = nullptr, *dgp = nullptr, *dgr = nullptr, *out = nullptr;
+    int rc = 0;
+    if (grad_cap64) rc |= dalloc(c, &dgc, cg);
+    if (grad_capgrey) rc |= dalloc(c, &dgp, nc);
+    if (grad_capross) rc |= dalloc(c, &dgr, nc);
+    const size_t nout = (size_t)nc * (3 + 30 + std::max(nion, 1));
+    rc |= dalloc(c, &out, nout);
+    if (rc) return rc;
+    if (dgc) HIPCHK(hipMemcpyAsync(dgc, grad_cap64, cg * 8, hipMemcpyHostToDevice, c->stream));
+    if (dgp) HIPCHK(hipMemcpyAsync(dgp, grad_capgrey, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (dgr) HIPCHK(hipMemcpyAsync(dgr, grad_capross, nc * 8, hipMemcpyHostToDevice, c->stream));
+    double *gT = out, *gn = out + nc, *ge = out + 2 * nc, *gf = out + 3 * nc, *go = out + 33 * (size_t)nc;
+    launch_opacity_vjp(c, c->opac_flags, dgc, dgp, dgr, gT, gn, ge, gf, go, ff_prefactor(),
+                       rgg_scratch(c));
+    HIPCHK(hipGetLastError());
+    if (out_grad_temp) HIPCHK(hipMemcpyAsync(out_grad_temp, gT, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_grad_natom) HIPCHK(hipMemcpyAsync(out_grad_natom, gn, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_grad_nelec) HIPCHK(hipMemcpyAsync(out_grad_nelec, ge, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_grad_natomfr) HIPCHK(hipMemcpyAsync(out_grad_natomfr, gf, (size_t)30 * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_grad_oc && nion > 0) HIPCHK(hipMemcpyAsync(out_grad_oc, go, (size_t)nion * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    rc = tm.stop();
+    dfree(c, dgc); dfree(c, dgp); dfree(c, dgr); dfree(c, out);
+    return rc;
 }
 
 int slgpu_sample_groups(slgpu_ctx *c, int table, int direct, int64_t n, const double *r,

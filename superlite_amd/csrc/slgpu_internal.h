@@ -80,6 +80,12 @@ struct Dev {
     int *pig, *pix, *perr;
     int64_t *pnstep;
     // tallies
+    // inputs of the last opacity build kept for its backward, and the backward's scratch
+    double *natom_in, *vol_in, *nelec_in; // nc
+    double *ginvT;                        // nion*nc (ion-major)
+    double *gcapT;                        // ng*nc cell-fastest dL/dcap after the means
+    double *gcell;                        // 4*nc: dL/dT (means), dL/dsig, dL/dhckt, dL/dhlparr
+    double *gfr, *goc;                    // 30*nc, nion*nc element- / ion-major
     double *tally, *jrad, *tot;           // nc, jrad_copies*nc*ng, 8
     int jrad_copies;                      // replicas of grd_jrad that spread atomic traffic
     unsigned long long *ct;               // 10
@@ -112,6 +118,7 @@ struct slgpu_ctx {
     std::vector<int16_t> h_liz, h_lii;
     std::vector<float> h_gxs, h_chilw;
     std::vector<double> h_emit_cache;
+    int opac_flags = 0;         // flags of the last opacity build (its backward uses them)
     std::vector<void *> allocations;
     double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[16];
@@ -126,6 +133,11 @@ void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
                       int flags);
 void launch_opacity(slgpu_ctx *c, int flags);
 void launch_opacity_finish(slgpu_ctx *c);
+double ff_prefactor();
+const double *rgg_scratch(slgpu_ctx *c);
+void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double *gp,
+                        const double *gr, double *gT, double *gnat, double *gne, double *gfr,
+                        double *goc, double ffc1, const double *rgg);
 void launch_ddmc_tables(slgpu_ctx *c);
 void launch_ddmc_probs(slgpu_ctx *c);
 void launch_sources(slgpu_ctx *c, uint32_t iteration);

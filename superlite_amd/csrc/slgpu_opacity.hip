@@ -105,6 +105,7 @@ __global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const doubl
         double o = oc[(size_t)i * d.nion + k];
         d.ocT[(size_t)k * nc + i] = o;
         d.ocg[(size_t)k * nc + i] = o * ginv[(size_t)i * d.nion + k];
+        d.ginvT[(size_t)k * nc + i] = ginv[(size_t)i * d.nion + k];
     }
     if (k < d.nel) {
         int z = d.el_z[k];
@@ -291,11 +292,19 @@ void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
                        d_natom, d_vol, d_nelec, d_oc, d_ginv, flags, g_rgg_scratch(c));
 }
 
+// free-free prefactor of physical_opacity.f:51-52 (cgs)
+double ff_prefactor()
+{
+    return 4.0 * (PC_E * PC_E * PC_E * PC_E * PC_E * PC_E) /
+           (3.0 * PC_H * PC_ME * (PC_C * PC_C * PC_C * PC_C)) *
+           std::sqrt(PC_PI2 / (3 * PC_ME * PC_H * PC_C));
+}
+
+const double *rgg_scratch(slgpu_ctx *c) { return g_rgg_scratch(c); }
+
 void launch_opacity(slgpu_ctx *c, int flags)
 {
-    const double ffc1 = 4.0 * (PC_E * PC_E * PC_E * PC_E * PC_E * PC_E) /
-                        (3.0 * PC_H * PC_ME * (PC_C * PC_C * PC_C * PC_C)) *
-                        std::sqrt(PC_PI2 / (3 * PC_ME * PC_H * PC_C));
+    const double ffc1 = ff_prefactor();
     dim3 grid((c->nx + 63) / 64, (c->ng + 3) / 4);
     if (c->d.precision == SLGPU_MIXED)
         hipLaunchKernelGGL(k_opacity<true>, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,

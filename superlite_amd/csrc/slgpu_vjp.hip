@@ -1,0 +1,245 @@
+// Backward of the opacity build (differentiation form) on gfx950: the vector-Jacobian
+// product of the FP64 group opacity cap(g,i) (GAS/physical_opacity.f:93-247), the Planck
+// mean (GAS/opacity_planckmean.f:18-24) and the Rosseland mean (GAS/opacity_rossmean.f:18-21)
+// with respect to the build's own inputs: temperature, natom, n_e, the mass fractions and
+// the ion populations oc (ginv, V and the masses are constants of the build).
+//
+//   k_vjp_means     one block per cell: the means' gradients folded into dL/dcap, plus
+//                   their direct T and sigma_Thomson terms
+//   k_vjp_opacity   one thread per (cell, group), 64 cells x 16 groups per block: the bb
+//                   line terms recomputed (not stored), bf and ff, with per-ion and
+//                   per-element accumulators in LDS flushed once per block
+//   k_vjp_finish    one thread per cell: chain rule through hckt = hc/kT, the ff prefactor
+//                   (natom/V)^2 n_e and sigma_T = c_T n_e natom/V
+//
+// The means are taken through the FP32-rounded cap as the forward evaluates them, with the
+// rounding treated as the identity.  Elements absent from every cell are skipped by the
+// forward's ff sum, so their mass-fraction gradient is returned as 0.
+#include "slgpu_device.h"
+#include "slgpu_internal.h"
+
+namespace slgpu {
+
+constexpr double LN10 = 2.302585092994045684;
+constexpr int VJP_GROUPS = 16;          // groups per block (4 per wave)
+
+// d/dx of x^3/(e^x-1) and of x^4 e^x/(e^x-1)^2, zero where the forward caps x at 300
+__device__ __forceinline__ double d_fplanck(double x)
+{
+    if (x >= 300.0) return 0.0;
+    const double e = exp(x), em = e - 1.0;
+    return (3.0 * x * x * em - x * x * x * e) / (em * em);
+}
+
+__device__ __forceinline__ double d_fdplanck(double x)
+{
+    if (x >= 300.0) return 0.0;
+    const double e = exp(x), em = e - 1.0;
+    const double f = (x * x * x * x * e) / (em * em);
+    return f * (4.0 / x + 1.0 - 2.0 * e / em);
+}
+
+// d specintv_g / dT and d spectintv_g / dT (x = hc/(kT lambda) scales as 1/T)
+__device__ __forceinline__ double d_specintv_dT(double tinv, const double *__restrict__ wlinv, int ig)
+{
+    const double a = HCK * tinv;
+    const double x0 = a * wlinv[ig - 1], x1 = a * wlinv[ig];
+    const double dx = fabs(x1 - x0), m = 0.5 * (x1 + x0);
+    return -(FTPI4 * dx * tinv) * (fplanck(m) + m * d_fplanck(m));
+}
+
+__device__ __forceinline__ double d_spectintv_dT(double tinv, const double *__restrict__ wlinv, int ig)
+{
+    const double a = HCK * tinv;
+    const double x0 = a * wlinv[ig - 1], x1 = a * wlinv[ig];
+    const double dx = fabs(x1 - x0), m = 0.5 * (x1 + x0);
+    const double v = FTPI4T * dx * fdplanck(m);
+    if (v != v) return 0.0;                 // the forward's NaN guard makes this group 0
+    const double r = -(FTPI4T * dx * tinv) * (fdplanck(m) + m * d_fdplanck(m));
+    return r != r ? 0.0 : r;
+}
+
+__device__ __forceinline__ double block_sum256_v(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_vjp_means(Dev d, const double *__restrict__ gc,
+                                                   const double *__restrict__ gp,
+                                                   const double *__restrict__ gr)
+{
+    __shared__ double sh[4];
+    const int i = blockIdx.x, ng = d.ng, nc = d.nx;
+    const double tinv = d.tempinv[i], sg = d.sig[i], R = d.capross[i];
+    const double wp = gp ? gp[i] : 0.0, wr = gr ? gr[i] : 0.0;
+    const bool massive = d.mass[i] > 0.0;
+    double dT = 0.0, dsig = 0.0;
+    for (int g = threadIdx.x + 1; g <= ng; g += 256) {
+        const double c = (double)d.cap32[(size_t)i * ng + g - 1];
+        const double s = specintv_g(tinv, d.wlinv, g);
+        const double t = spectintv_g(tinv, d.wlinv, g);
+        const double w = sg + c;
+        double G = gc ? gc[(size_t)i * ng + g - 1] : 0.0;
+        G += wp * s;
+        dT += wp * c * d_specintv_dT(tinv, d.wlinv, g);
+        if (w > 0.0 && R != 0.0) {
+            // capross = 1/sum_g t_g/(sig + cap_g)
+            G += wr * R * R * t / (w * w);
+            dsig += wr * R * R * t / (w * w);
+            dT -= wr * R * R * d_spectintv_dT(tinv, d.wlinv, g) / w;
+        }
+        d.gcapT[(size_t)(g - 1) * nc + i] = massive ? G : 0.0;
+    }
+    dT = block_sum256_v(dT, sh);
+    dsig = block_sum256_v(dsig, sh);
+    if (threadIdx.x == 0) {
+        d.gcell[0 * nc + i] = dT;
+        d.gcell[1 * nc + i] = dsig;
+        d.gcell[2 * nc + i] = 0.0;
+        d.gcell[3 * nc + i] = 0.0;
+    }
+}
+
+// dynamic LDS: s_oc[nion][64] then s_fr[nel][64]
+__global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ffc1,
+                                                     const double *__restrict__ rgg_tab)
+{
+    extern __shared__ double s_acc[];
+    const int nc = d.nx, ng = d.ng, nion = d.nion, nel = d.nel;
+    double *s_oc = s_acc, *s_fr = s_acc + (size_t)nion * 64;
+    for (int k = threadIdx.x; k < (nion + nel) * 64; k += 256) s_acc[k] = 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const bool act = i < nc;
+    const int ic = act ? i : nc - 1;
+    const double hckt = d.hckt[ic];
+    double dh_acc = 0.0, dhl_acc = 0.0;
+    for (int q = 0; q < VJP_GROUPS / 4; ++q) {
+        const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * VJP_GROUPS + wave * (VJP_GROUPS / 4) + q) + 1;
+        if (g > ng) break;
+        const double G = act ? d.gcapT[(size_t)(g - 1) * nc + i] : 0.0;
+        const double wlo = d.wl[g - 1], whi = d.wl[g];
+        if (!(flags & SLGPU_NOBB)) {
+            const double wlc = 0.5 * (wlo + whi);
+            const double wlinv = 1.0 / wlc;
+            const double dwl = whi - wlo;
+            const double ec = exp(-hckt * wlinv);
+            const double expfac = 1.0 - ec;
+            const double phi = wlc * wlc / (dwl * PC_C);
+            const int l0 = d.gstart[g], l1 = d.gstart[g + 1];
+            const double *ocg = d.ocg + ic;
+            double s0 = 0.0, s1 = 0.0;
+            for (int l = l0; l < l1; ++l) {
+                const LineRec r = d.lines[l];
+                const double o = ocg[(size_t)r.ion * nc];
+                if (o > 0.0) {
+                    const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
+                    const double term = base * o;
+                    s0 += term;
+                    s1 += term * (double)r.chilw;
+                    atomicAdd(&s_oc[r.ion * 64 + lane], G * base * d.ginvT[(size_t)r.ion * nc + ic]);
+                }
+            }
+            dh_acc += G * (-s1 + s0 * wlinv * ec / expfac);
+        }
+        if (!(flags & SLGPU_NOBF)) {
+            const double *xs = d.bfxs + (size_t)(g - 1) * nion;
+            for (int k = 0; k < nion; ++k) {
+                const double x = xs[k];
+                if (x != 0.0) atomicAdd(&s_oc[k * 64 + lane], G * x);
+            }
+        }
+        if (!(flags & SLGPU_NOFF)) {
+            const double wl = wlo;
+            const double u = hckt * (1.0 / wl);
+            const double ru = 10.0 * (log10(u) + 4.0) + 1.0;
+            int iu = (int)floor(ru);
+            iu = iu < 1 ? 1 : (iu > FF_NU - 1 ? FF_NU - 1 : iu);
+            const double du = ru - iu;
+            const double eu = exp(-u);
+            const double hnh = ffc1 * sqrt(hckt) * (1.0 - eu) * (wl * wl * wl);   // help / hlparr
+            const double help = hnh * d.hlparr[ic];
+            const double dhelp = help * (0.5 / hckt + eu * (1.0 / wl) / (1.0 - eu));
+            const double dru = 10.0 / (hckt * LN10), drgg = 5.0 / (hckt * LN10);
+            for (int k = 0; k < nel; ++k) {
+                const int z = d.el_z[k];
+                const double rgg = rgg_tab[(size_t)k * nc + ic];
+                int igg = (int)floor(rgg);
+                igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
+                const double dgg = rgg - igg;
+                const double *T = d.gff;
+                const double t00 = T[(igg - 1) * FF_NU + (iu - 1)], t10 = T[(igg - 1) * FF_NU + iu];
+                const double t01 = T[igg * FF_NU + (iu - 1)], t11 = T[igg * FF_NU + iu];
+                double gff = (1.0 - du) * (1.0 - dgg) * t00 + du * (1.0 - dgg) * t10 +
+                             (1.0 - du) * dgg * t01 + du * dgg * t11;
+                double dg = ((1.0 - dgg) * (t10 - t00) + dgg * (t11 - t01)) * dru +
+                            ((1.0 - du) * (t01 - t00) + du * (t11 - t10)) * drgg;
+                if (rgg > FF_NGG && !(gff > 1.0)) { gff = 1.0; dg = 0.0; }
+                const double zz = (double)(z * z);
+                const double nf = d.natomfr[(size_t)ic * 30 + (z - 1)];
+                dh_acc += G * (dhelp * gff + help * dg) * zz * nf;
+                dhl_acc += G * hnh * gff * zz * nf;
+                atomicAdd(&s_fr[k * 64 + lane], G * help * gff * zz);
+            }
+        }
+    }
+    if (act) {
+        atomicAdd(&d.gcell[2 * nc + i], dh_acc);
+        atomicAdd(&d.gcell[3 * nc + i], dhl_acc);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (nion + nel) * 64; k += 256) {
+        const int row = k >> 6, cell = blockIdx.x * 64 + (k & 63);
+        const double v = s_acc[k];
+        if (cell < nc && v != 0.0) {
+            if (row < nion) atomicAdd(&d.goc[(size_t)row * nc + cell], v);
+            else atomicAdd(&d.gfr[(size_t)(row - nion) * nc + cell], v);
+        }
+    }
+}
+
+// chain rule to the inputs; outputs in the forward's input layouts
+__global__ __launch_bounds__(256) void k_vjp_finish(Dev d, int flags, double *gT, double *gnat,
+                                                    double *gne, double *gfr_out, double *goc_out)
+{
+    constexpr double cthomson = 8.0 * PC_PI * (PC_E * PC_E * PC_E * PC_E) /
+                                (3.0 * (PC_ME * PC_ME) * (PC_C * PC_C * PC_C * PC_C));
+    const int nc = d.nx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double T = d.temp[i], hckt = d.hckt[i];
+    const double natom = d.natom_in[i], vol = d.vol_in[i], ne = d.nelec_in[i];
+    const double gsig = d.gcell[1 * nc + i], ghl = d.gcell[3 * nc + i];
+    const bool thomson = !(flags & SLGPU_NOTHOMSON) && d.mass[i] > 0.0;
+    gT[i] = d.gcell[0 * nc + i] + d.gcell[2 * nc + i] * (-hckt / T);
+    const double nv = natom / vol;
+    gne[i] = ghl * nv * nv + (thomson ? gsig * cthomson * natom / vol : 0.0);
+    gnat[i] = ghl * 2.0 * natom * ne / (vol * vol) + (thomson ? gsig * cthomson * ne / vol : 0.0);
+    for (int z = 0; z < 30; ++z) gfr_out[(size_t)i * 30 + z] = 0.0;
+    for (int k = 0; k < d.nel; ++k) gfr_out[(size_t)i * 30 + d.el_z[k] - 1] = d.gfr[(size_t)k * nc + i];
+    for (int k = 0; k < d.nion; ++k) goc_out[(size_t)i * d.nion + k] = d.goc[(size_t)k * nc + i];
+}
+
+void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double *gp,
+                        const double *gr, double *gT, double *gnat, double *gne, double *gfr,
+                        double *goc, double ffc1, const double *rgg)
+{
+    const int nc = c->nx, ng = c->ng;
+    (void)hipMemsetAsync(c->d.goc, 0, (size_t)std::max(c->nion, 1) * nc * 8, c->stream);
+    (void)hipMemsetAsync(c->d.gfr, 0, (size_t)30 * nc * 8, c->stream);
+    hipLaunchKernelGGL(k_vjp_means, dim3(nc), dim3(256), 0, c->stream, c->d, gc, gp, gr);
+    const size_t lds = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
+    dim3 grid((nc + 63) / 64, (ng + VJP_GROUPS - 1) / VJP_GROUPS);
+    hipLaunchKernelGGL(k_vjp_opacity, grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    hipLaunchKernelGGL(k_vjp_finish, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d,
+                       flags, gT, gnat, gne, gfr, goc);
+}
+
+}  // namespace slgpu

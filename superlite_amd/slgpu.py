@@ -28,7 +28,7 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_set_groups", "slgpu_set_lines", "slgpu_set_bf_tables", "slgpu_set_ff_table",
            "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
-           "slgpu_timers", "slgpu_stream")
+           "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream")
 
 
 class SlgpuError(RuntimeError):
@@ -74,6 +74,7 @@ def lib():
         L.slgpu_fluxtally.argtypes = [_vp, _vp, _vp, _vp]
         L.slgpu_get_particles.argtypes = [_vp, _i64] + [_vp] * 8
         L.slgpu_sample_groups.argtypes = [_vp, _i, _i, _i64, _vp, _vp, _vp]
+        L.slgpu_opacity_vjp.argtypes = [_vp] + [_vp] * 8
         L.slgpu_timers.argtypes = [_vp, _vp, _i, _i]
         L.slgpu_stream.restype = _vp
         L.slgpu_stream.argtypes = [_vp]
@@ -139,6 +140,7 @@ class Context:
     def set_ions(self, ion_iz, ion_ii):
         iz, ii = _c(ion_iz, np.int32), _c(ion_ii, np.int32)
         self._check(self.L.slgpu_set_ions(self.h, len(iz), _p(iz), _p(ii)))
+        self.nion = len(iz)
 
     # -- per iteration ----------------------------------------------------
     def opacity_build(self, temp, natom, vol, mass, nelec, natomfr, oc, ginv, flags=0,
@@ -210,6 +212,19 @@ class Context:
                                                             ("stat", "x", "mu", "wl", "e", "ig", "nstep", "err")]))
         return pp
 
+    def opacity_vjp(self, grad_cap64=None, grad_capgrey=None, grad_capross=None):
+        """dL/d(T, natom, n_e, natomfr, oc) of the last opacity_build for the given
+        dL/d(cap64 (nc, ng), capgrey (nc), capross (nc)); natomfr (30, nc), oc (nc, nion)."""
+        nc, ng = self.nx, self.ng
+        keep = [None if g is None else _c(g, np.float64) for g in (grad_cap64, grad_capgrey, grad_capross)]
+        out = dict(temp=np.zeros(nc), natom=np.zeros(nc), nelec=np.zeros(nc),
+                   natomfr=np.zeros((nc, 30)), oc=np.zeros((nc, max(self.nion, 1))))
+        self._check(self.L.slgpu_opacity_vjp(self.h, *[_p(k) for k in keep],
+                                             *[_p(out[k]) for k in ("temp", "natom", "nelec", "natomfr", "oc")]))
+        out["natomfr"] = out["natomfr"].T.copy()
+        out["oc"] = out["oc"][:, : self.nion]
+        return out
+
     def sample_groups(self, table, r, cell, direct=False):
         """group sampled by table (0-3 lumped L/R/D/S, 4 emitgroup) for draws r in cells"""
         r = np.ascontiguousarray(r, np.float64)
@@ -223,7 +238,7 @@ class Context:
         ms = np.zeros(8)
         self._check(self.L.slgpu_timers(self.h, _p(ms), 8, int(reset)))
         return dict(opacity=ms[0], prep=ms[1], sources=ms[2], transport=ms[3], fluxtally=ms[4],
-                    opacity_kernel=ms[5], transport_kernel=ms[6])
+                    opacity_kernel=ms[5], transport_kernel=ms[6], opacity_vjp=ms[7])
 
 
 def setup_workload(w, st, wl, lines, tables, precision=SLGPU_FP64, seed=1, npartmax=None,

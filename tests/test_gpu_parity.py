@@ -206,6 +206,111 @@ def test_precision_form_c1(S, tables):
         assert abs(t["estart"] - (t["eout"] + t["evelo"] + t["edead"])) < 1e-12 * t["estart"]
 
 
+def _perturbed(st, kind, idx, h):
+    s = dict(st)
+    a = np.array(st[kind], dtype=np.float64, copy=True)
+    a[idx] *= 1.0 + h
+    s[kind] = a
+    return s, st[kind][idx] * h
+
+
+def _fd_check(ctx, st, flags, probes, Wc, h=1e-4):
+    """central differences of sum(Wc*cap64), taken element by element, against the VJP"""
+    base = gpu_opacity(ctx, st, flags)
+    g = ctx.opacity_vjp(grad_cap64=Wc)
+    worst = 0.0
+    for kind, idx in probes:
+        sp, dx = _perturbed(st, kind, idx, h)
+        sm, _ = _perturbed(st, kind, idx, -h)
+        fd = float(np.sum(Wc * (gpu_opacity(ctx, sp, flags)["cap64"] -
+                                gpu_opacity(ctx, sm, flags)["cap64"]))) / (2 * dx)
+        an = g[kind][idx]
+        err = abs(fd - an) / max(abs(fd), abs(an), 1e-300)
+        worst = max(worst, err)
+        say(f"flags {flags} {kind:8s} {str(idx):12s} fd {fd: .6e} vjp {an: .6e} rel {err:.2e}")
+    return worst, g, base
+
+
+def test_opacity_vjp_central_differences(S, tables):
+    """Backward of the opacity build against central differences (h = 1e-4 relative) of
+    the GPU FP64 forward and of the oracle's FP64 forward.  Each input kind is differenced
+    where its term is resolvable in FP64: the free-free inputs (natom, n_e, mass fractions)
+    on ff-only builds, since ff is <= 1e-10 of the c1 opacity; the populations on ions
+    holding at least 1e-3 of their cell's largest population; temperature with every term
+    on, with bb+bf only and with ff only.  The means take the FP32-rounded gas_cap as the
+    reference does, a step function that differences cannot resolve, so they are
+    differenced on temperature, whose Planck-weight dependence is smooth (h = 1e-3), to
+    1e-3: the staircase still shows in probes whose derivative nearly cancels."""
+    say("case: opacity backward c1")
+    ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1024)
+    rng = np.random.default_rng(21)
+    nc, ng = w.nx, w.ng
+    temps = [("temp", (int(i),)) for i in rng.choice(nc, 6, replace=False)]
+    oc = st["oc"]
+    big = np.argwhere(oc >= 1e-3 * oc.max(axis=1, keepdims=True))
+    ocs = [("oc", tuple(int(v) for v in big[k])) for k in rng.choice(len(big), 8, replace=False)]
+    present = [z for z in range(30) if np.any(st["natomfr"][z] != 0)]
+    ffs = [("natom", (int(i),)) for i in rng.choice(nc, 4, replace=False)]
+    ffs += [("nelec", (int(i),)) for i in rng.choice(nc, 4, replace=False)]
+    ffs += [("natomfr", (int(rng.choice(present)), int(i))) for i in rng.choice(nc, 6, replace=False)]
+    checks = [(0, temps + ocs), (S.NOFF, temps[:3]), (S.NOBB | S.NOBF, temps[:3] + ffs)]
+    worst = 0.0
+    for flags, probes in checks:
+        base = gpu_opacity(ctx, st, flags)
+        Wc = rng.standard_normal((nc, ng)) / np.abs(base["cap64"][base["cap64"] > 0]).mean()
+        wv, g_cap, _ = _fd_check(ctx, st, flags, probes, Wc)
+        worst = max(worst, wv)
+        if flags == 0:
+            g0, W0 = g_cap, Wc
+    say("worst rel error (group opacity)", worst)
+    assert worst < 1e-4
+    # the means, temperature probes
+    base = gpu_opacity(ctx, st)
+    Wp = rng.standard_normal(nc) / np.abs(base["capgrey"]).mean()
+    Wr = rng.standard_normal(nc) / np.abs(base["capross"]).mean()
+    g_mean = ctx.opacity_vjp(grad_capgrey=Wp, grad_capross=Wr)
+    worst_m = 0.0
+    for kind, idx in temps:
+        sp, dx = _perturbed(st, kind, idx, 1e-3)
+        sm, _ = _perturbed(st, kind, idx, -1e-3)
+        op, om = gpu_opacity(ctx, sp), gpu_opacity(ctx, sm)
+        fd = (float(np.sum(Wp * (op["capgrey"] - om["capgrey"]))) +
+              float(np.sum(Wr * (op["capross"] - om["capross"])))) / (2 * dx)
+        an = g_mean[kind][idx]
+        err = abs(fd - an) / max(abs(fd), abs(an), 1e-300)
+        worst_m = max(worst_m, err)
+        say(f"means {kind} {idx} fd {fd: .6e} vjp {an: .6e} rel {err:.2e}")
+    # the FP32 gas_cap staircase inside the means limits these differences to ~1e-4
+    assert worst_m < 1e-3
+    # the oracle's FP64 forward differenced on temperature and population probes
+    for kind, idx in temps[:2] + ocs[:2]:
+        caps = []
+        dx = st[kind][idx] * 1e-4
+        for sgn in (1, -1):
+            sp, _ = _perturbed(st, kind, idx, sgn * 1e-4)
+            o = po.opacity(wl, sp["temp"], sp["natom"], sp["vol"], sp["mass"], sp["nelec"],
+                           sp["natomfr"], sp["ion_iz"], sp["ion_ii"], sp["oc"], sp["ginv"], lines, tables)
+            caps.append(o["cap64"])
+        fd = float(np.sum(W0 * (caps[0] - caps[1]))) / (2 * dx)
+        err = abs(fd - g0[kind][idx]) / max(abs(fd), 1e-300)
+        say(f"oracle {kind} {idx}: fd {fd:.6e} vjp {g0[kind][idx]:.6e} rel {err:.2e}")
+        assert err < 1e-4
+    Wc = W0
+    g_cap = g0
+    gpu_opacity(ctx, st)
+    # the autograd.Function gives the same gradient
+    import torch
+    from superlite_amd import autograd as sag
+    consts = dict(vol=st["vol"], mass=st["mass"], ginv=st["ginv"])
+    ts = {k: torch.tensor(np.asarray(st[k], np.float64), requires_grad=True)
+          for k in ("temp", "natom", "nelec", "natomfr", "oc")}
+    cap, grey, ross = sag.opacity(ctx, consts, ts["temp"], ts["natom"], ts["nelec"], ts["natomfr"], ts["oc"])
+    (cap * torch.from_numpy(Wc)).sum().backward()
+    np.testing.assert_allclose(ts["temp"].grad.numpy(), g_cap["temp"], rtol=1e-12)
+    np.testing.assert_allclose(ts["oc"].grad.numpy(), g_cap["oc"], rtol=1e-12, atol=0)
+    ctx.close()
+
+
 def test_sources_c1_per_particle(S, tables):
     say("case: sources c1")
     ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 14, seed=5)
