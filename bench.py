@@ -141,6 +141,43 @@ def precision_errors(m, f):
     }
 
 
+def opacity_grad(args):
+    """Differentiation form: the opacity build and its backward (slgpu_opacity_vjp) at the
+    workload's size, timed with the library's HIP-event phase timers."""
+    from superlite_amd import slgpu, synth
+    w, st, wl, lines = synth.build(args.workload)
+    tables = synth.load_tables()
+    ctx = slgpu.setup_workload(w, st, wl, lines, tables, npartmax=1024, seed=2024)
+    rng = np.random.default_rng(7)
+    gcap = rng.standard_normal((w.nx, w.ng))
+    ones = np.ones(w.nx)
+    gas = (st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"], st["oc"], st["ginv"])
+    for _ in range(args.warmup):
+        ctx.opacity_build(*gas, want=())
+        ctx.opacity_vjp(gcap, ones, ones)
+    ctx.timers(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.opacity_build(*gas, want=())
+        ctx.opacity_vjp(gcap, ones, ones)
+    dt = time.perf_counter() - t0
+    tm = ctx.timers()
+    ctx.close()
+    K = args.steps
+    cg = w.nx * w.ng
+    fwd, bwd = tm["opacity"] / K, tm["opacity_vjp"] / K
+    print(json.dumps({
+        "metric": "opacity cell·groups/s, forward and backward (differentiation form)",
+        "value": cg / (bwd * 1e-3), "unit": "cell·groups/s (backward)",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": dt / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {w.nx} shells x {w.ng} groups x {w.nline} lines, "
+                               "forward + VJP w.r.t. T, natom, n_e, mass fractions, populations"},
+        "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
+        "forward_cell_groups_per_s": cg / (fwd * 1e-3)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +189,8 @@ def main():
     ap.add_argument("--warmup-n2s", type=int, default=16,
                     help="log2 particles of the warmup iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="transport", choices=["transport", "opacity-grad"],
+                    help="opacity-grad: time the opacity build and its backward instead")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one elementwise copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -160,6 +199,9 @@ def main():
                                    "but not built (run with --gpus 1)"}))
         sys.exit(2)
 
+    if args.mode == "opacity-grad":
+        opacity_grad(args)
+        return
     if args.calibration_copy_mib > 0:
         calibration_copy(args.calibration_copy_mib)
     from superlite_amd import slgpu, synth

@@ -608,14 +608,12 @@ __device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
 // 271-292 at the end), then takes another, so lanes stay busy until the pool is empty.
 // Lanes that leave the loop drop out of the votes; a pass always runs the type of at
 // least one remaining lane, so every particle advances.
-// optional occupancy target of the transport kernel (waves per SIMD), a build-time knob
-#ifdef SLGPU_TRANSPORT_WAVES
-#define SLGPU_TRANSPORT_ATTR __attribute__((amdgpu_waves_per_eu(SLGPU_TRANSPORT_WAVES, 8)))
-#else
-#define SLGPU_TRANSPORT_ATTR
-#endif
+// Occupancy target per precision (waves per SIMD), measured on c2: the precision form runs
+// best at 4 (VGPRs capped at 128), the FP64 form at 3 (capped at 168); the compiler's own
+// choice (2 and 3) is 6-11% slower.
 template <bool M>
-__global__ __launch_bounds__(256) SLGPU_TRANSPORT_ATTR void k_transport(Dev d, uint32_t iteration)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M ? 4 : 3, 8)))
+void k_transport(Dev d, uint32_t iteration)
 {
     extern __shared__ double s_tally[];
     const int ng = d.ng, nx = d.nx;
