@@ -133,6 +133,28 @@ module slgpu_iface
       type(c_ptr), value :: stat, x, mu, wl, e, ig, nstep, err
     end function
 
+    ! differentiation form: adjoint of the last slgpu_opacity_build (any pointer may be
+    ! c_null_ptr): gradients in, gradients w.r.t. T, natom, n_e, natomfr, oc out
+    integer(c_int) function slgpu_opacity_vjp(ctx, grad_cap64, grad_capgrey, grad_capross, &
+         grad_temp, grad_natom, grad_nelec, grad_natomfr, grad_oc) &
+         bind(C, name="slgpu_opacity_vjp")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: grad_cap64, grad_capgrey, grad_capross
+      type(c_ptr), value :: grad_temp, grad_natom, grad_nelec, grad_natomfr, grad_oc
+    end function
+
+    integer(c_int) function slgpu_sample_groups(ctx, table, direct, n, r, cell, group) &
+         bind(C, name="slgpu_sample_groups")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: table, direct
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: r(*)
+      integer(c_int), intent(in) :: cell(*)
+      integer(c_int), intent(out) :: group(*)
+    end function
+
     integer(c_int) function slgpu_timers(ctx, ms, n, reset) bind(C, name="slgpu_timers")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
