@@ -1288,6 +1288,7 @@ int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result
             int perr = 0;
             while (p.stat == 0) {
                 istep++;
+                or_stream_step(&st, (uint32_t)istep);
                 int icold = p.ic, igold = p.ig;
                 double erd, jr;
                 int ierr;

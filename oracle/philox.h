@@ -42,11 +42,13 @@ static inline double or_u01(uint32_t a, uint32_t b)
     return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-/* Per-particle stream: key = (seed, iteration), counter = (particle, block, phase, 0).
- * Draw k of a stream uses block k/2, words (0,1) for even k and (2,3) for odd k. */
+/* Per-particle stream: key = (seed, iteration).  Sequential mode (sources): draw k uses
+ * counter (particle, k/2, phase, 0).  Stepped mode (transport): draw j of step s uses
+ * counter (particle, s, phase, j/2).  Words (0,1) serve even draws, (2,3) odd ones. */
 typedef struct {
     uint32_t k0, k1, pid, phase;
-    uint32_t ndraw;
+    uint32_t ndraw, step;
+    int stepped;
     or_u32x4 buf;
 } or_stream;
 
@@ -54,6 +56,13 @@ static inline void or_stream_init(or_stream *s, uint32_t seed, uint32_t iter,
                                   uint32_t pid, uint32_t phase)
 {
     s->k0 = seed; s->k1 = iter; s->pid = pid; s->phase = phase; s->ndraw = 0;
+    s->step = 0; s->stepped = 0;
+}
+
+/* start transport step `step` (1-based event count of the particle) */
+static inline void or_stream_step(or_stream *s, uint32_t step)
+{
+    s->stepped = 1; s->step = step; s->ndraw = 0;
 }
 
 static inline double or_draw(or_stream *s)
@@ -61,6 +70,7 @@ static inline double or_draw(or_stream *s)
     uint32_t k = s->ndraw++;
     if ((k & 1u) == 0) {
         or_u32x4 c = {{s->pid, k >> 1, s->phase, 0u}};
+        if (s->stepped) { c.v[1] = s->step; c.v[3] = k >> 1; }
         s->buf = or_philox4x32_10(c, s->k0, s->k1);
         return or_u01(s->buf.v[0], s->buf.v[1]);
     }

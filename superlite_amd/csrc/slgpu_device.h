@@ -124,23 +124,28 @@ __device__ __forceinline__ double planck(double wl, double temp)
 }
 
 // ---------------------------------------------------------------------------
-// Counter-based RNG: Philox-4x32-10 (Salmon et al., SC'11).  One stream per particle:
-// key (seed, iteration), counter (particle, block, phase, 0); draw k uses block k/2,
-// word pair (0,1) for even k and (2,3) for odd k; uniform strictly inside (0,1).
+// Counter-based RNG: Philox-4x32-10 (Salmon et al., SC'11).  One stream per particle, key
+// (seed, iteration).  Sequential mode (sources): draw k uses counter (particle, k/2, phase,
+// 0).  Stepped mode (transport): draw j of step s uses counter (particle, s, phase, j/2),
+// and begin_step computes the step's first block at the top of the step, where every lane
+// of the pass is active, instead of wherever a lane's draw count happens to be even.
+// Words (0,1) serve even draws and (2,3) odd ones; uniforms lie in (0,1].
 struct Philox {
-    uint32_t k0, k1, pid, phase, ndraw;
-    uint32_t w2, w3;
+    uint32_t k0, k1, pid, phase, ndraw, step;
+    uint32_t w0, w1, w2, w3;
+    bool stepped;
 
     __device__ __forceinline__ void init(uint32_t seed, uint32_t iter, uint32_t p, uint32_t ph)
     {
-        k0 = seed; k1 = iter; pid = p; phase = ph; ndraw = 0; w2 = w3 = 0;
+        k0 = seed; k1 = iter; pid = p; phase = ph; ndraw = 0; step = 0; stepped = false;
+        w0 = w1 = w2 = w3 = 0;
     }
 
     __device__ __forceinline__ static void block(uint32_t c0, uint32_t c1, uint32_t c2,
                                                  uint32_t c3, uint32_t k0, uint32_t k1,
                                                  uint32_t out[4])
     {
-        // rolled: one copy of the round per draw site keeps the transport kernel's code
+        // rolled: one copy of the round per call site keeps the transport kernel's code
         // small enough for the instruction cache
         #pragma unroll 1
         for (int r = 0; r < 10; ++r) {
@@ -160,25 +165,27 @@ struct Philox {
         return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
     }
 
-    // continue a stream after n draws (re-derives the buffered half block when n is odd)
-    __device__ __forceinline__ void resume(uint32_t n)
+    __device__ __forceinline__ void fill(uint32_t j)
     {
-        ndraw = n;
-        if (n & 1u) {
-            uint32_t o[4];
-            block(pid, n >> 1, phase, 0u, k0, k1, o);
-            w2 = o[2]; w3 = o[3];
-        }
+        uint32_t o[4];
+        if (stepped) block(pid, step, phase, j, k0, k1, o);
+        else block(pid, j, phase, 0u, k0, k1, o);
+        w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
+    }
+
+    // start transport step s with its first block computed
+    __device__ __forceinline__ void begin_step(uint32_t s)
+    {
+        stepped = true; step = s; ndraw = 0;
+        fill(0u);
     }
 
     __device__ __forceinline__ double draw()
     {
-        uint32_t k = ndraw++;
+        const uint32_t k = ndraw++;
         if ((k & 1u) == 0) {
-            uint32_t o[4];
-            block(pid, k >> 1, phase, 0u, k0, k1, o);
-            w2 = o[2]; w3 = o[3];
-            return u01(o[0], o[1]);
+            if (!(stepped && k == 0)) fill(k >> 1);
+            return u01(w0, w1);
         }
         return u01(w2, w3);
     }

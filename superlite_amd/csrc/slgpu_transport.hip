@@ -12,7 +12,8 @@
 // dominated by effective scatters, so the group samplers they call are the exact bucket
 // maps of slgpu_sample.h: one table load for most draws instead of nested searches.
 // Per-particle randomness is a Philox-4x32-10 stream keyed on (seed, iteration) and
-// counted on (particle, draw, phase), so trajectories do not depend on the schedule.
+// counted on (particle, step, phase, draw within the step), so trajectories do not depend
+// on the schedule, and every lane of a pass computes its step's first block together.
 //
 // Tallies: the per-cell energy tally lives in LDS (nc doubles) per block and is flushed
 // with FP64 atomics at the end; consecutive events of one particle in the same
@@ -664,6 +665,7 @@ void k_transport(Dev d, uint32_t iteration)
         const int n_imc = __popcll(__ballot(imc)), n_ddmc = __popcll(__ballot(!imc));
         if (imc != (n_imc >= n_ddmc)) continue;
         istep++;
+        s.begin_step((uint32_t)istep);
         const int icold = p.ix, igold = p.ig;
         double erd, jrd;
         int ierr;
