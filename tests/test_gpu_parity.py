@@ -107,6 +107,28 @@ def test_opacity_c2_full_size(S, tables):
     ctx.close()
 
 
+def test_opacity_c5_full_size(S, tables):
+    """1000 shells with the dense CSM shell (8000-9000 km/s, rho x1e3, T to 5e4 K), 10^4
+    groups, 4e6 lines; the oracle on 22 shells including the CSM ones"""
+    say("case: opacity c5")
+    ctx, w, st, wl, lines = make_ctx(S, "c5", tables, npartmax=1024)
+    g = gpu_opacity(ctx, st)
+    cells = np.unique(np.concatenate([np.arange(0, w.nx, 50), [272, 285, 298]]))
+    sub = {k: st[k][cells] for k in ("temp", "natom", "vol", "mass", "nelec", "oc", "ginv")}
+    o = po.opacity(wl, sub["temp"], sub["natom"], sub["vol"], sub["mass"], sub["nelec"],
+                   st["natomfr"][:, cells], st["ion_iz"], st["ion_ii"], sub["oc"], sub["ginv"],
+                   lines, tables)
+    m = o["cap64"] != 0
+    rel = np.abs(g["cap64"][cells][m] - o["cap64"][m]) / np.abs(o["cap64"][m])
+    say("c5 cap64 max rel vs oracle", rel.max(), "T range", sub["temp"].min(), sub["temp"].max())
+    assert np.array_equal(g["cap64"][cells] != 0, m)
+    assert rel.max() < 1e-12
+    assert ulp_diff(g["cap"][cells], o["cap"]).max() <= 1
+    np.testing.assert_allclose(g["capgrey"][cells], o["capgrey"], rtol=1e-12)
+    np.testing.assert_allclose(g["capross"][cells], o["capross"], rtol=1e-12)
+    ctx.close()
+
+
 def oracle_model(st, w, wl, g, prep=None, nvol=None, **kw):
     m = po.Model(st["xarr"], st["vxarr"], st["divv"], st["vol"], wl, int(st["nep"]),
                  int(st["nepg"]), g["cap"], g["sig"], g["capgrey"], 1.0 / st["temp"], st["temp"],
@@ -135,7 +157,7 @@ def test_ddmc_prep_c1(S, tables):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["c1", "c2"])
+@pytest.mark.parametrize("name", ["c1", "c2", "c5"])
 def test_sampler_tables_exact(S, tables, name):
     """The bucket maps the transport samples groups with return exactly what the plain
     searches return, for random draws and for draws on and next to every kind of bucket
