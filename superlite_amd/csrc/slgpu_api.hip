@@ -160,6 +160,8 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, (double **)&d.vxarr, nc + 1);
     rc |= dalloc(c, (double **)&d.divv, nc);
     rc |= dalloc(c, (double **)&d.vol, nc);
+    rc |= dalloc(c, (double **)&d.dvdr, nc);
+    rc |= dalloc(c, (double **)&d.c3divv, nc);
     rc |= dalloc(c, (double **)&d.wl, ng + 1);
     rc |= dalloc(c, (double **)&d.wlinv, ng + 1);
     rc |= dalloc(c, (double **)&d.gff, 81 * 41);
@@ -269,6 +271,15 @@ int slgpu_set_grid(slgpu_ctx *c, const double *xarr, const double *vxarr, const 
     HIPCHK(hipMemcpyAsync((void *)c->d.vxarr, vxarr, (nx + 1) * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync((void *)c->d.divv, divv, nx * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync((void *)c->d.vol, vol, nx * 8, hipMemcpyHostToDevice, c->stream));
+    // per-cell constants of the transport, with the kernels' own expression order (one
+    // IEEE operation each, so the values are the ones the step would compute)
+    std::vector<double> dvdr(nx), c3(nx);
+    for (int i = 0; i < nx; ++i) {
+        dvdr[i] = (vxarr[i + 1] - vxarr[i]) / (xarr[i + 1] - xarr[i]);
+        c3[i] = (3.0 * PC_C) / divv[i];
+    }
+    HIPCHK(hipMemcpyAsync((void *)c->d.dvdr, dvdr.data(), nx * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void *)c->d.c3divv, c3.data(), nx * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_grid = true;
     return 0;

@@ -42,6 +42,7 @@ struct Dev {
     uint32_t seed;
     // mesh (nx+1 / nx)
     const double *xarr, *vxarr, *divv, *vol;
+    const double *dvdr, *c3divv;  // nx: (v_{i+1}-v_i)/dx_i and 3c/divv_i (transport11.f90:130,145-149)
     // groups (ng+1)
     const double *wl, *wlinv;
     // lines

@@ -119,10 +119,17 @@ template <bool M> __device__ __forceinline__ double p_log(double x)
     if constexpr (M) return (double)__logf((float)x);
     else return log(x);
 }
-template <bool M> __device__ __forceinline__ double p_exp(double x)
+// -ln(r)/k: a sampled path length for opacity k (transport11.f90:112-125)
+template <bool M> __device__ __forceinline__ double p_path(double r, double k)
 {
-    if constexpr (M) return (double)__expf((float)x);
-    else return exp(x);
+    if constexpr (M) return -(double)__logf((float)r) / k;
+    else return -log(r) / k;
+}
+// exp(-divv*t/3): the DDMC adiabatic factor (diffusion11.f90:623-629)
+template <bool M> __device__ __forceinline__ double p_adiabatic(double divv, double t)
+{
+    if constexpr (M) return (double)__expf((float)(-divv * t) * (1.0f / 3.0f));
+    else return exp(-divv * t / 3);
 }
 // r^3-uniform radius in [xl, xr]: (r*xr^3 + (1-r)*xl^3)**(1.0/3.0) with the reference's
 // single-precision exponent (diffusion11.f90:336,597).  Radii cubed (~1e47 cm^3) overflow
@@ -161,11 +168,11 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     }
     if (db != db) return 1;
     const double sg = SIGT(ic), cp = CAPT(ig, ic);
-    if (sg > 0.0) { r1 = s.draw(); dthm = -p_log<M>(r1) / (elabfact * sg); }
+    if (sg > 0.0) { r1 = s.draw(); dthm = p_path<M>(r1, elabfact * sg); }
     else dthm = far;
     if (cp <= 0.0) dcol = far;
-    else { r1 = s.draw(); dcol = -p_log<M>(r1) / (elabfact * cp); }
-    const double dvdr = (VXA(ix + 1) - VXA(ix)) / DXX(ix);
+    else { r1 = s.draw(); dcol = p_path<M>(r1, elabfact * cp); }
+    const double dvdr = d.dvdr[ix - 1];
     const double vhelp = VXA(ix) - dvdr * xl;
     if (fabs(vx - vhelp) / PC_C >= 0.1) {
         // Newton-Raphson Doppler distance (transport11.f90:159-199)
@@ -192,8 +199,8 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         }
         if (ddop <= 0.0 || it == 100) ddop = far;
     } else {
-        if (dvdr > 0 && ig < ng) ddop = 3.0 * PC_C / DIVV(ix) * (elabfact - wl * WLI(ig + 1));
-        else if (dvdr < 0 && ig > 1) ddop = 3.0 * PC_C / DIVV(ix) * (elabfact - wl * WLI(ig));
+        if (dvdr > 0 && ig < ng) ddop = d.c3divv[ix - 1] * (elabfact - wl * WLI(ig + 1));
+        else if (dvdr < 0 && ig > 1) ddop = d.c3divv[ix - 1] * (elabfact - wl * WLI(ig));
         else ddop = far;
         if (ddop < 0.0) ddop = far;
     }
@@ -565,7 +572,7 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         if (x < XA(1)) {
             p.stat = 2;                                        // :623-626
         } else {
-            const double f = p_exp<M>(-DIVV(ix) * ddmct / 3);
+            const double f = p_adiabatic<M>(DIVV(ix), ddmct);
             evelo = evelo + e * (1.0 - f);
             e = e * f;
             e0 = e0 * f;
