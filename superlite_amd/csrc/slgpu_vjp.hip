@@ -13,8 +13,10 @@
 //                   (natom/V)^2 n_e and sigma_T = c_T n_e natom/V
 //
 // The means are taken through the FP32-rounded cap as the forward evaluates them, with the
-// rounding treated as the identity.  Elements absent from every cell are skipped by the
-// forward's ff sum, so their mass-fraction gradient is returned as 0.
+// rounding treated as the identity.  In the precision form (SLGPU_MIXED) the recomputed
+// bb line terms are FP32 with Kahan-compensated sums, like that form's forward; bf, ff,
+// the means and every accumulator stay FP64.  Elements absent from every cell are skipped
+// by the forward's ff sum, so their mass-fraction gradient is returned as 0.
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
 
@@ -106,7 +108,18 @@ __global__ __launch_bounds__(256) void k_vjp_means(Dev d, const double *__restri
     }
 }
 
-// dynamic LDS: s_oc[nion][64] then s_fr[nel][64]
+// Kahan-compensated FP32 accumulation (precision form of the line sums)
+__device__ __forceinline__ void kahan_add(float &sum, float &comp, float v)
+{
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// dynamic LDS: s_oc[nion][64] then s_fr[nel][64].  M: the precision form, whose bb line
+// terms are recomputed in FP32 with Kahan-compensated sums as in its forward.
+template <bool M>
 __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ffc1,
                                                      const double *__restrict__ rgg_tab)
 {
@@ -136,15 +149,33 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
             const int l0 = d.gstart[g], l1 = d.gstart[g + 1];
             const double *ocg = d.ocg + ic;
             double s0 = 0.0, s1 = 0.0;
-            for (int l = l0; l < l1; ++l) {
-                const LineRec r = d.lines[l];
-                const double o = ocg[(size_t)r.ion * nc];
-                if (o > 0.0) {
-                    const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
-                    const double term = base * o;
-                    s0 += term;
-                    s1 += term * (double)r.chilw;
-                    atomicAdd(&s_oc[r.ion * 64 + lane], G * base * d.ginvT[(size_t)r.ion * nc + ic]);
+            if constexpr (M) {
+                const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
+                float a0 = 0.0f, c0 = 0.0f, a1 = 0.0f, c1 = 0.0f;
+                for (int l = l0; l < l1; ++l) {
+                    const LineRec r = d.lines[l];
+                    const double o = ocg[(size_t)r.ion * nc];
+                    if (o > 0.0) {
+                        const float base = phi32 * r.gxs * expf(-(r.chilw * hk32)) * ef32;
+                        const float term = base * (float)o;
+                        kahan_add(a0, c0, term);
+                        kahan_add(a1, c1, term * r.chilw);
+                        atomicAdd(&s_oc[r.ion * 64 + lane], G * (double)base * d.ginvT[(size_t)r.ion * nc + ic]);
+                    }
+                }
+                s0 = (double)a0;
+                s1 = (double)a1;
+            } else {
+                for (int l = l0; l < l1; ++l) {
+                    const LineRec r = d.lines[l];
+                    const double o = ocg[(size_t)r.ion * nc];
+                    if (o > 0.0) {
+                        const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
+                        const double term = base * o;
+                        s0 += term;
+                        s1 += term * (double)r.chilw;
+                        atomicAdd(&s_oc[r.ion * 64 + lane], G * base * d.ginvT[(size_t)r.ion * nc + ic]);
+                    }
                 }
             }
             dh_acc += G * (-s1 + s0 * wlinv * ec / expfac);
@@ -237,7 +268,10 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     hipLaunchKernelGGL(k_vjp_means, dim3(nc), dim3(256), 0, c->stream, c->d, gc, gp, gr);
     const size_t lds = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
     dim3 grid((nc + 63) / 64, (ng + VJP_GROUPS - 1) / VJP_GROUPS);
-    hipLaunchKernelGGL(k_vjp_opacity, grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    if (c->d.precision == SLGPU_MIXED)
+        hipLaunchKernelGGL(k_vjp_opacity<true>, grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    else
+        hipLaunchKernelGGL(k_vjp_opacity<false>, grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
     hipLaunchKernelGGL(k_vjp_finish, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d,
                        flags, gT, gnat, gne, gfr, goc);
 }

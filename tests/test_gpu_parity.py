@@ -333,6 +333,30 @@ def test_opacity_vjp_central_differences(S, tables):
     ctx.close()
 
 
+def test_opacity_vjp_precision_form(S, tables):
+    """The precision form's backward (FP32 Kahan line terms) against the FP64 backward on
+    the same inputs and output gradients: every input gradient within 1e-5 of the largest
+    FP64 gradient of its kind."""
+    say("case: opacity backward, precision form c1")
+    rng = np.random.default_rng(5)
+    res = {}
+    for prec in (S.SLGPU_FP64, S.SLGPU_MIXED):
+        ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1024, precision=prec)
+        base = gpu_opacity(ctx, st)
+        if prec == S.SLGPU_FP64:
+            Wc = rng.standard_normal(base["cap64"].shape) / np.abs(base["cap64"]).mean()
+            Wp = rng.standard_normal(w.nx) / np.abs(base["capgrey"]).mean()
+            Wr = rng.standard_normal(w.nx) / np.abs(base["capross"]).mean()
+        res[prec] = ctx.opacity_vjp(Wc, Wp, Wr)
+        ctx.close()
+    f, m = res[S.SLGPU_FP64], res[S.SLGPU_MIXED]
+    for k in ("temp", "natom", "nelec", "natomfr", "oc"):
+        scale = np.abs(f[k]).max()
+        err = np.abs(m[k] - f[k]).max() / scale
+        say(f"{k}: max |mixed - fp64| / max|fp64| = {err:.2e}")
+        assert err < 1e-5, k
+
+
 def test_sources_c1_per_particle(S, tables):
     say("case: sources c1")
     ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 14, seed=5)
