@@ -195,7 +195,7 @@ def main():
                     help="run one elementwise copy of this size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed "
+        print(json.dumps({"error": "superlite_amd runs on one GPU; multi-GPU sharding is designed "
                                    "but not built (run with --gpus 1)"}))
         sys.exit(2)
 
