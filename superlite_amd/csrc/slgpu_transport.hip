@@ -619,6 +619,15 @@ __device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
 // Occupancy target per precision (waves per SIMD), measured on c2: the precision form runs
 // best at 4 (VGPRs capped at 128), the FP64 form at 3 (capped at 168); the compiler's own
 // choice (2 and 3) is 6-11% slower.
+// A wave runs an IMC pass when n_imc * SLGPU_VOTE_IMC >= n_ddmc * SLGPU_VOTE_DDMC.  IMC
+// lanes wait out DDMC passes, so letting them go once they are a third of the live lanes
+// (2:1) beat a plain majority by 3-4% on c2 (1:2 and 4:1 were slower).
+#ifndef SLGPU_VOTE_IMC
+#define SLGPU_VOTE_IMC 2
+#endif
+#ifndef SLGPU_VOTE_DDMC
+#define SLGPU_VOTE_DDMC 1
+#endif
 template <bool M>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M ? 4 : 3, 8)))
 void k_transport(Dev d, uint32_t iteration)
@@ -670,7 +679,7 @@ void k_transport(Dev d, uint32_t iteration)
         // hold; lanes of the other type wait a pass, so each pass executes one body.
         const bool imc = p.itype == 1 || d.puretran;
         const int n_imc = __popcll(__ballot(imc)), n_ddmc = __popcll(__ballot(!imc));
-        if (imc != (n_imc >= n_ddmc)) continue;
+        if (imc != (n_imc * SLGPU_VOTE_IMC >= n_ddmc * SLGPU_VOTE_DDMC)) continue;
         istep++;
         s.begin_step((uint32_t)istep);
         const int icold = p.ix, igold = p.ig;
