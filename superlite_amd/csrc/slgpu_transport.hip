@@ -675,8 +675,8 @@ void k_transport(Dev d, uint32_t iteration)
             istep = 0;
             have = true;
         }
-        // One event.  The wave runs one event type per pass, the one most of its lanes
-        // hold; lanes of the other type wait a pass, so each pass executes one body.
+        // One event.  The wave runs one event type per pass, chosen by the vote above the
+        // kernel; lanes of the other type wait a pass, so each pass executes one body.
         const bool imc = p.itype == 1 || d.puretran;
         const int n_imc = __popcll(__ballot(imc)), n_ddmc = __popcll(__ballot(!imc));
         if (imc != (n_imc * SLGPU_VOTE_IMC >= n_ddmc * SLGPU_VOTE_DDMC)) continue;
