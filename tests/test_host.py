@@ -65,3 +65,14 @@ def test_fortran_interface_compiles(tmp_path):
     r = subprocess.run([fc, "-c", src, "-module-dir", str(tmp_path), "-o", str(tmp_path / "i.o")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_fortran_interface_binds_every_entry_point():
+    """every function declared in include/slgpu.h has a bind(C) interface of that name"""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "slgpu.h")).read()
+    declared = set(re.findall(r"\b(slgpu_[a-z_0-9]+)\s*\(", hdr))
+    f90 = open(os.path.join(ROOT, "superlite_amd", "fortran", "slgpu_iface.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(slgpu_[a-z_0-9]+)"\)', f90))
+    assert declared, "no declarations found"
+    assert declared <= bound, sorted(declared - bound)
