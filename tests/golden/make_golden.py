@@ -16,7 +16,7 @@ Outputs
   ref_edge_opacity.npz  a hand-built 3-cell / 5-group case with lines on, below and above the
                         group edges, and the reference's FP32 opacities for it
 
-Usage:  python tests/golden/make_golden.py [--mc]
+Usage:  python tests/golden/make_golden.py [--mc] [--mc-c2]
 """
 from __future__ import annotations
 
@@ -59,7 +59,12 @@ def main():
     ap.add_argument("--mc", action="store_true", help="also run the reference Monte Carlo")
     ap.add_argument("--mc-n2s", type=int, default=14)
     ap.add_argument("--mc-iter", type=int, default=16)
+    ap.add_argument("--mc-c2", action="store_true",
+                    help="only the c2 reference Monte Carlo (2^10 particles, 12 iterations)")
     args = ap.parse_args()
+    if args.mc_c2:
+        mc_case("c2", 1.0, 10, 12, workload="c2")
+        return
 
     s0 = RefSession()
     ph1, ph2, gff = s0.tables()
@@ -144,10 +149,10 @@ def edge_case():
     print("edge case done", flush=True)
 
 
-def mc_case(case, scale, n2s, niter):
+def mc_case(case, scale, n2s, niter, workload="c1"):
     """Reference transport statistics: niter independent iterations of 2^n2s particles."""
-    w = synth.WORKLOADS["c1"]
-    st = dict(np.load(os.path.join(OUT, "state_c1.npz")))
+    w = synth.WORKLOADS[workload]
+    st = dict(np.load(os.path.join(OUT, f"state_{workload}.npz")))
     s = RefSession()
     g = s.setup(w.nx, w.ng, w.wlmin, w.wlmax, st["xarr"], st["vxarr"], n2s=n2s,
                 nomp=os.cpu_count() or 1)
@@ -157,7 +162,17 @@ def mc_case(case, scale, n2s, niter):
               st["ion_iz"], st["ion_ii"], st["oc"] * scale, st["ginv"])
     lines = synth.line_list(w, st["ion_iz"], st["ion_ii"])
     s.set_lines(*lines)
-    op = s.opacity(flags=0 if scale == 1.0 else 4)
+    if workload == "c1":
+        op = s.opacity(flags=0 if scale == 1.0 else 4)
+    else:
+        # the reference's bb loop races on cap when threaded (physical_opacity.f:104-156), so
+        # the multithreaded transport session takes the oracle's single-pass opacity, which
+        # matches the reference's own to <= 1 ulp in gas_cap
+        from oracle import pyoracle as po
+        op = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"] * scale,
+                        st["natomfr"], st["ion_iz"], st["ion_ii"], st["oc"] * scale, st["ginv"],
+                        lines, synth.load_tables(), flags=0 if scale == 1.0 else 4)
+        s.put_opacity(op["cap"], op["sig"], op["capgrey"], op["capross"])
     p = s.prep()
     keys = ("luminos", "lumnum", "tally", "eout", "evelo", "nstepimc", "nstepddmc",
             "nmethswap", "nflux")
@@ -168,6 +183,7 @@ def mc_case(case, scale, n2s, niter):
             rows[k].append(r[k])
         print(case, "iteration", it, "eout", r["eout"], "steps", r["nstepimc"], r["nstepddmc"], flush=True)
     np.savez_compressed(os.path.join(OUT, f"ref_{case}_mc.npz"), n2s=n2s, scale=scale,
+                        workload=workload,
                         flags=0 if scale == 1.0 else 4, cap=op["cap"], sig=op["sig"],
                         capgrey=op["capgrey"], opaclump=p["opaclump"], emitprob=p["emitprob"],
                         emit=p["emit"], **{k: np.array(v) for k, v in rows.items()})

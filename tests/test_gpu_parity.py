@@ -432,20 +432,21 @@ def coarse(a, nb=20):
     return a[..., :n].reshape(a.shape[:-1] + (nb, -1)).sum(-1)
 
 
-@pytest.mark.parametrize("case", ["c1mixed", "c1"])
+@pytest.mark.parametrize("case", ["c1mixed", "c1", "c2"])
 def test_transport_statistics_vs_reference(S, tables, golden, case):
     """MC tallies within 3 sigma of SuperLite's own Monte Carlo (golden per-iteration rows)."""
     say("case: transport statistics", case)
     g = golden(f"ref_{case}_mc.npz")
     scale = float(g["scale"])
     n2s = int(g["n2s"])
-    w, st, wl, lines = synth.build("c1")
+    workload = str(g["workload"]) if "workload" in g else "c1"
+    w, st, wl, lines = synth.build(workload)
     st = dict(st)
     st["oc"] = st["oc"] * scale
     st["nelec"] = st["nelec"] * scale
     ctx = S.setup_workload(w, st, wl, lines, tables, npartmax=1 << n2s, seed=21)
     rows = []
-    for it in range(24):
+    for it in range(24 if workload == "c1" else 16):
         r = S.iteration(ctx, st, 1 << n2s, it + 1, w.lbol, flags=int(g["flags"]))
         rows.append(dict(luminos=r["flux"]["luminos"], tally=r["transport"]["tally"],
                          eout=r["transport"]["eout"], evelo=r["transport"]["evelo"]))
