@@ -436,6 +436,8 @@ def coarse(a, nb=20):
 def test_transport_statistics_vs_reference(S, tables, golden, case):
     """MC tallies within 3 sigma of SuperLite's own Monte Carlo (golden per-iteration rows)."""
     say("case: transport statistics", case)
+    if not os.path.exists(os.path.join(os.path.dirname(__file__), "golden", f"ref_{case}_mc.npz")):
+        pytest.skip(f"ref_{case}_mc.npz not generated (tests/golden/make_golden.py --mc-c2)")
     g = golden(f"ref_{case}_mc.npz")
     scale = float(g["scale"])
     n2s = int(g["n2s"])
