@@ -243,6 +243,7 @@ def main():
                                f"{w.nline} lines, 2^{w.n2s} particles/iteration",
                    "precision": "mixed" if mixed else "fp64"},
         "opacity_cell_groups_per_s": opac_rate,
+        "opacity_kernel_cell_groups_per_s": w.nx * w.ng * K / (tm["opacity_kernel"] * 1e-3),
         "transport_errors": R["nerr"],
         "phase_ms_per_step": {k: v / K for k, v in tm.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
