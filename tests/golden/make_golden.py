@@ -60,10 +60,10 @@ def main():
     ap.add_argument("--mc-n2s", type=int, default=14)
     ap.add_argument("--mc-iter", type=int, default=16)
     ap.add_argument("--mc-c2", action="store_true",
-                    help="only the c2 reference Monte Carlo (2^10 particles, 12 iterations)")
+                    help="only the c2 reference Monte Carlo (2^10 particles, 8 iterations)")
     args = ap.parse_args()
     if args.mc_c2:
-        mc_case("c2", 1.0, 10, 12, workload="c2")
+        mc_case("c2", 1.0, 10, 8, workload="c2")
         return
 
     s0 = RefSession()
@@ -182,11 +182,13 @@ def mc_case(case, scale, n2s, niter, workload="c1"):
         for k in keys:
             rows[k].append(r[k])
         print(case, "iteration", it, "eout", r["eout"], "steps", r["nstepimc"], r["nstepddmc"], flush=True)
-    np.savez_compressed(os.path.join(OUT, f"ref_{case}_mc.npz"), n2s=n2s, scale=scale,
-                        workload=workload,
-                        flags=0 if scale == 1.0 else 4, cap=op["cap"], sig=op["sig"],
-                        capgrey=op["capgrey"], opaclump=p["opaclump"], emitprob=p["emitprob"],
-                        emit=p["emit"], **{k: np.array(v) for k, v in rows.items()})
+        # written after every iteration, so a long run keeps what it has finished
+        np.savez_compressed(os.path.join(OUT, f"ref_{case}_mc.npz"), n2s=n2s, scale=scale,
+                            workload=workload,
+                            flags=0 if scale == 1.0 else 4, cap=op["cap"], sig=op["sig"],
+                            capgrey=op["capgrey"], opaclump=p["opaclump"],
+                            emitprob=p["emitprob"], emit=p["emit"],
+                            **{k: np.array(v) for k, v in rows.items()})
     s.close()
 
 
