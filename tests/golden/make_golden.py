@@ -182,12 +182,14 @@ def mc_case(case, scale, n2s, niter, workload="c1"):
         for k in keys:
             rows[k].append(r[k])
         print(case, "iteration", it, "eout", r["eout"], "steps", r["nstepimc"], r["nstepddmc"], flush=True)
-        # written after every iteration, so a long run keeps what it has finished
+        # written after every iteration, so a long run keeps what it has finished; the c1
+        # files also keep the iteration's opacity and DDMC tables (small at c1)
+        extra = {}
+        if workload == "c1":
+            extra = dict(cap=op["cap"], sig=op["sig"], capgrey=op["capgrey"],
+                         opaclump=p["opaclump"], emitprob=p["emitprob"], emit=p["emit"])
         np.savez_compressed(os.path.join(OUT, f"ref_{case}_mc.npz"), n2s=n2s, scale=scale,
-                            workload=workload,
-                            flags=0 if scale == 1.0 else 4, cap=op["cap"], sig=op["sig"],
-                            capgrey=op["capgrey"], opaclump=p["opaclump"],
-                            emitprob=p["emitprob"], emit=p["emit"],
+                            workload=workload, flags=0 if scale == 1.0 else 4, **extra,
                             **{k: np.array(v) for k, v in rows.items()})
     s.close()
 
