@@ -11,6 +11,11 @@ per second of whole-step wall time, for the FP32-mixed precision form by default
 path is timed beside it on the same inputs and seeds, and the line carries the mixed-vs-FP64
 differences of the opacities, shell tallies, escaped energy and spectrum.  The opacity rate
 (cell-groups per second of the opacity-build phase) is reported beside it.
+
+One iteration at the default workload (400 shells, 10^4 groups, 2e6 lines, 2^24 particles)
+takes tens of seconds, so --steps is an upper bound: each precision times whole iterations
+until its share of --budget-s would be exceeded (at least one), and `steps` reports the
+iterations actually timed beside `steps_requested`.
 """
 from __future__ import annotations
 
@@ -28,11 +33,12 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(w, st, wl, lines, tables, target_s=15.0):
-    """The oracle (CPU FP64 restatement, OpenMP) on a bounded sample of the same workload."""
+def cpu_baseline(w, st, wl, lines, tables, target_s=12.0):
+    """The oracle (CPU FP64 restatement, OpenMP) on a bounded sample of the same workload:
+    the opacity over a subset of shells, then the transport of a particle sample sized from
+    a first small run so the whole baseline stays near target_s of CPU time."""
     from oracle import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # opacity: a subset of cells over the full line list and group grid
     cells = np.arange(0, w.nx, max(1, w.nx // 8))
     t0 = time.perf_counter()
     op = po.opacity(wl, st["temp"][cells], st["natom"][cells], st["vol"][cells], st["mass"][cells],
@@ -40,7 +46,6 @@ def cpu_baseline(w, st, wl, lines, tables, target_s=15.0):
                     st["oc"][cells], st["ginv"][cells], lines, tables)
     t_op = time.perf_counter() - t0
     opac_rate = len(cells) * w.ng / t_op
-    # transport: full opacity for the model, then a small particle sample
     full = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"],
                       st["natomfr"], st["ion_iz"], st["ion_ii"], st["oc"], st["ginv"], lines,
                       tables)
@@ -50,9 +55,8 @@ def cpu_baseline(w, st, wl, lines, tables, target_s=15.0):
     m.set(opaclump=m.leakage(), emitprob=m.emission_probability())
     emit, _ = po.sourceenergy(st["rcell"], st["temp"], w.lbol)
     m.set(emit=emit)
-    n = 64
-    spent, steps = 0.0, 0
-    while spent < target_s and n <= (1 << 16):
+    n, spent = 32, 0.0
+    while True:
         m.set(nvol=po.sourcenumbers(emit, n))
         t0 = time.perf_counter()
         r = m.iteration(77, 1)
@@ -60,11 +64,15 @@ def cpu_baseline(w, st, wl, lines, tables, target_s=15.0):
         spent += dt
         steps = r["nstepimc"] + r["nstepddmc"]
         rate = steps / dt
-        sample = f"{n} particles of the workload ({steps} particle-steps)"
-        n *= 4
+        sample = f"{n} particles of the workload ({steps} particle-steps, {dt:.1f} s)"
+        left = target_s - spent
+        if left < 2.0 * dt or n >= (1 << 16):
+            break
+        n = min(1 << 16, n * min(32, 1 << int(np.log2(max(2.0, left / dt)))))
     return dict(value=rate, unit="particle-steps/s", cores=threads, kind="port",
                 sample=sample, opacity_cell_groups_per_s=opac_rate,
-                opacity_sample=f"{len(cells)} of {w.nx} cells x {w.ng} groups x {w.nline} lines")
+                opacity_sample=f"{len(cells)} of {w.nx} cells x {w.ng} groups x {w.nline} lines",
+                note="FP64 oracle restatement on the host cores; a baseline, not a target")
 
 
 def measured_traffic(kernel, workload):
@@ -92,8 +100,9 @@ def calibration_copy(mib):
     del x, y
 
 
-def run_precision(slgpu, w, st, wl, lines, tables, prec, args):
-    """Warm up, then time args.steps whole iterations at the workload's particle count."""
+def run_precision(slgpu, w, st, wl, lines, tables, prec, args, budget_s):
+    """Warm up, then time whole iterations at the workload's particle count: at most
+    args.steps, and no more than fit in budget_s of wall time (at least one)."""
     nstot = 1 << w.n2s
     ctx = slgpu.setup_workload(w, st, wl, lines, tables, precision=prec, seed=2024,
                                npartmax=nstot, errorfatal=False)
@@ -103,21 +112,26 @@ def run_precision(slgpu, w, st, wl, lines, tables, prec, args):
         print(f"# {name} warmup {k}: steps {r['transport']['nstepimc'] + r['transport']['nstepddmc']}",
               file=sys.stderr, flush=True)
     ctx.timers(reset=True)
-    steps, nerr, last = 0, 0, None
+    steps, nerr, last, done = 0, 0, None, 0
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        r = slgpu.iteration(ctx, st, nstot, k + 1, w.lbol, want=True)
+    while done < args.steps:
+        r = slgpu.iteration(ctx, st, nstot, done + 1, w.lbol, want=True)
         tr = r["transport"]
         steps += tr["nstepimc"] + tr["nstepddmc"]
         nerr += tr["nerr"]
         last = r
-        print(f"# {name} step {k}: particle-steps {tr['nstepimc'] + tr['nstepddmc']} "
+        done += 1
+        el = time.perf_counter() - t0
+        print(f"# {name} step {done - 1}: particle-steps {tr['nstepimc'] + tr['nstepddmc']} "
               f"(imc {tr['nstepimc']}, ddmc {tr['nstepddmc']}), max/particle {tr['nstepmax']}, "
-              f"escaped {tr['nflux']}, errors {tr['nerr']}", file=sys.stderr, flush=True)
+              f"escaped {tr['nflux']}, errors {tr['nerr']}, elapsed {el:.1f} s",
+              file=sys.stderr, flush=True)
+        if el + el / done > budget_s:
+            break
     dt = time.perf_counter() - t0
     tm = ctx.timers()
     ctx.close()
-    return dict(steps=steps, dt=dt, tm=tm, nerr=nerr, last=last)
+    return dict(steps=steps, iters=done, dt=dt, tm=tm, nerr=nerr, last=last)
 
 
 def precision_errors(m, f):
@@ -181,11 +195,15 @@ def opacity_grad(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="most iterations timed per precision (the time budget may stop earlier)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="both: value is the FP32-mixed form, timed beside FP64 on the same seeds")
+    ap.add_argument("--budget-s", type=float, default=330.0,
+                    help="wall-clock budget of the timed iterations, shared by the precisions "
+                         "(55%% to the first); each times at least one iteration")
     ap.add_argument("--warmup-n2s", type=int, default=16,
                     help="log2 particles of the warmup iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -195,8 +213,8 @@ def main():
                     help="run one elementwise copy of this size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "superlite_amd runs on one GPU; multi-GPU sharding is designed "
-                                   "but not built (run with --gpus 1)"}))
+        print(json.dumps({"error": "superlite_amd runs on one GPU; the sharded path is designed "
+                                   "but not built (run with --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.mode == "opacity-grad":
@@ -210,30 +228,41 @@ def main():
     nstot = 1 << w.n2s
     print(f"# workload {args.workload}: nx={w.nx} ng={w.ng} lines={w.nline} "
           f"particles=2^{w.n2s}", file=sys.stderr, flush=True)
+    exp32, exp64 = slgpu.peak_rates()
+    print(f"# measured exponential throughput: v_exp_f32 {exp32:.3e}/s, FP64 exp {exp64:.3e}/s",
+          file=sys.stderr, flush=True)
     precs = {"both": [slgpu.SLGPU_MIXED, slgpu.SLGPU_FP64], "mixed": [slgpu.SLGPU_MIXED],
              "fp64": [slgpu.SLGPU_FP64]}[args.precision]
-    runs = {p: run_precision(slgpu, w, st, wl, lines, tables, p, args) for p in precs}
+    shares = [0.55, 0.45] if len(precs) == 2 else [1.0]
+    runs = {p: run_precision(slgpu, w, st, wl, lines, tables, p, args, args.budget_s * f)
+            for p, f in zip(precs, shares)}
     main_p = precs[0]
     R = runs[main_p]
-    K = args.steps
+    K = R["iters"]
     value = R["steps"] / R["dt"]
     tm = R["tm"]
-    opac_rate = w.nx * w.ng * K / (tm["opacity"] * 1e-3)
-    # dominant kernel: k_transport.  Algorithmic bytes per particle-step: the FP32 group
-    # opacity (4 B), the step's threshold record (32 B), one sampler-table entry (4 B) and
-    # the FP64 read-modify-write of the grd_jrad tally (16 B) = 56 B; per particle, its
-    # state read once and written once (5 FP64 in, 5 FP64 + status + group + cell + step
-    # count + error out = 40 + 40 + 1 + 4 + 4 + 8 + 4 B = 101 B)
-    t_kernel = tm["transport_kernel"] * 1e-3 / K
-    bytes_kernel = 56.0 * R["steps"] / K + 101.0 * nstot
-    achieved = bytes_kernel / t_kernel / 1e9
     mixed = main_p == slgpu.SLGPU_MIXED
     kname = "k_transport<true>" if mixed else "k_transport<false>"
+    # dominant kernel: k_transport.  Algorithmic bytes: per particle-step the FP32 group
+    # opacity gather (4 B) and the FP64 read-modify-write of the step's grd_tally and
+    # grd_jrad entries (2 x 16 B) = 36 B; per particle its state read once and written once,
+    # 2 x 56 B (FP32-mixed) or 2 x 80 B (FP64).  Lookup tables the implementation adds
+    # (thresholds, bucket maps) are not algorithmic bytes: they show in the measured traffic.
+    t_kernel = tm["transport_kernel"] * 1e-3 / K
+    state_b = 56.0 if mixed else 80.0
+    bytes_kernel = 36.0 * R["steps"] / K + 2.0 * state_b * nstot
+    achieved = bytes_kernel / t_kernel / 1e9
+    # opacity build: one exponential per (line, cell) in the line loop
+    nline_used = int(np.count_nonzero((lines[0] * 1e-8 >= wl[0]) & (lines[0] * 1e-8 < wl[-1])))
+    t_opk = tm["opacity_kernel"] * 1e-3 / K
+    line_evals = float(nline_used) * w.nx
+    exp_peak = exp32 if mixed else exp64
+    opac_alg_bytes = w.nx * w.ng * (8.0 + 4.0 + 4.0) + 16.0 * nline_used
     out = {
         "metric": "particle-steps/s + opacity cell·groups/s, FP32-mixed vs FP64 at ≤1% tally err",
         "value": value,
         "unit": "particle-steps/s",
-        "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "n_gpus": 1, "steps": K, "steps_requested": args.steps, "warmup": args.warmup,
         "ms_per_step": R["dt"] / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if mixed else "f64",
@@ -241,27 +270,41 @@ def main():
         "config": {"workload": f"{args.workload}: {w.nx} shells to {w.vmax / 1e5:.0f} km/s, "
                                f"{w.ng} groups {w.wlmin * 1e8:.0f}-{w.wlmax * 1e8:.0f} A, "
                                f"{w.nline} lines, 2^{w.n2s} particles/iteration",
-                   "precision": "mixed" if mixed else "fp64"},
-        "opacity_cell_groups_per_s": opac_rate,
-        "opacity_kernel_cell_groups_per_s": w.nx * w.ng * K / (tm["opacity_kernel"] * 1e-3),
+                   "precision": "mixed" if mixed else "fp64",
+                   "step": "one whole iteration: opacity build, DDMC prep, sources, transport, flux tally"},
+        "opacity_cell_groups_per_s": w.nx * w.ng * K / (tm["opacity"] * 1e-3),
+        "opacity_kernel_cell_groups_per_s": w.nx * w.ng / t_opk,
         "transport_errors": R["nerr"],
         "phase_ms_per_step": {k: v / K for k, v in tm.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(kname, args.workload),
-                     "kernel": kname},
+                     "kernel": kname, "kernel_ms": t_kernel * 1e3,
+                     "algorithmic_bytes": f"36 B/particle-step + 2x{state_b:.0f} B/particle",
+                     "limiter": "latency and divergence (lanes active, wave waits), not bandwidth"},
+        "roofline_opacity": {"bound": "exp", "kernel": "k_opacity<true>" if mixed else "k_opacity<false>",
+                             "achieved": line_evals / t_opk, "peak": exp_peak,
+                             "unit": "line evaluations/s vs measured " +
+                                     ("v_exp_f32/s" if mixed else "FP64 exp()/s"),
+                             "frac": line_evals / t_opk / exp_peak, "kernel_ms": t_opk * 1e3,
+                             "hbm_achieved_gbs": opac_alg_bytes / t_opk / 1e9,
+                             "hbm_frac": opac_alg_bytes / t_opk / 1e9 / PEAK_HBM_GBS},
+        "measured_peaks": {"exp32_per_s": exp32, "exp64_per_s": exp64},
     }
     if slgpu.SLGPU_FP64 in runs and mixed:
         F = runs[slgpu.SLGPU_FP64]
-        out["fp64"] = {"value": F["steps"] / F["dt"], "ms_per_step": F["dt"] / K * 1e3,
-                       "opacity_cell_groups_per_s": w.nx * w.ng * K / (F["tm"]["opacity"] * 1e-3),
+        KF = F["iters"]
+        out["fp64"] = {"value": F["steps"] / F["dt"], "steps": KF, "ms_per_step": F["dt"] / KF * 1e3,
+                       "opacity_cell_groups_per_s": w.nx * w.ng * KF / (F["tm"]["opacity"] * 1e-3),
+                       "opacity_kernel_cell_groups_per_s": w.nx * w.ng * KF / (F["tm"]["opacity_kernel"] * 1e-3),
                        "transport_errors": F["nerr"],
-                       "phase_ms_per_step": {k: v / K for k, v in F["tm"].items()}}
+                       "phase_ms_per_step": {k: v / KF for k, v in F["tm"].items()}}
         out["mixed_speedup_vs_fp64"] = value / out["fp64"]["value"]
         out["mixed_vs_fp64_errors"] = precision_errors(R, F)
     if not args.no_cpu_baseline:
+        print("# cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, st, wl, lines, tables)
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -176,6 +176,12 @@ int slgpu_timers(slgpu_ctx *ctx, double *ms, int n, int reset);
 /* the HIP stream all work of the context is ordered on (hipStream_t) */
 void *slgpu_stream(slgpu_ctx *ctx);
 
+/* Measurement hook, no reference counterpart: measured throughput of the current device's
+ * raw FP32 exponential (v_exp_f32) and of the FP64 exp() sequence, in evaluations per
+ * second.  These are the compute ceilings of the opacity build's line loop (one
+ * exponential per line and cell).  Either pointer may be NULL. */
+int slgpu_peak_rates(double *out_exp32_per_s, double *out_exp64_per_s);
+
 #ifdef __cplusplus
 }
 #endif

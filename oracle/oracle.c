@@ -146,6 +146,9 @@ double or_planck(double wl, double temp)
 /* bfxsmod.f:65-153 (Verner 1995/96 fits); ph1(6,7,30,30), ph2(7,30,30) Fortran order */
 #define PH1(p, s, z, e) ((double)ph1[(p - 1) + 6 * ((s - 1) + 7 * ((z - 1) + 30 * (e - 1)))])
 #define PH2(p, z, e) ((double)ph2[(p - 1) + 7 * ((z - 1) + 30 * (e - 1))])
+/* bf_ph1(6,..)**2, bf_ph2(5,..)**2, bf_ph2(7,..)**2 are real*4 expressions in bfxsmod.f:141,148,149:
+ * the square is rounded to single precision before it joins the double-precision sum */
+#define SQ32(v) ((double)((float)(v) * (float)(v)))
 
 double or_bfxs(int nz, int ne, double e, const float *ph1, const float *ph2)
 {
@@ -181,15 +184,15 @@ double or_bfxs(int nz, int ne, double e, const float *ph1, const float *ph2)
             double p1 = -PH1(5, is, nz, ne);
             double y = e / PH1(2, is, nz, ne);
             double q = -0.5 * p1 - ll[is - 1] - 5.5;
-            double a = PH1(3, is, nz, ne) * ((y - 1.0) * (y - 1.0) + PH1(6, is, nz, ne) * PH1(6, is, nz, ne));
+            double a = PH1(3, is, nz, ne) * ((y - 1.0) * (y - 1.0) + SQ32(PH1(6, is, nz, ne)));
             double b = sqrt(y / PH1(4, is, nz, ne)) + 1.0;
             xs = xs + a * pow(y, q) * pow(b, p1);
         } else {
             double p1 = -PH2(4, nz, ne);
             double q = -0.5 * p1 - 5.5;
             double x = e / PH2(1, nz, ne) - PH2(6, nz, ne);
-            double z = sqrt(x * x + PH2(7, nz, ne) * PH2(7, nz, ne));
-            double a = PH2(2, nz, ne) * ((x - 1.0) * (x - 1.0) + PH2(5, nz, ne) * PH2(5, nz, ne));
+            double z = sqrt(x * x + SQ32(PH2(7, nz, ne)));
+            double a = PH2(2, nz, ne) * ((x - 1.0) * (x - 1.0) + SQ32(PH2(5, nz, ne)));
             double b = 1.0 + sqrt(z / PH2(3, nz, ne));
             xs = xs + a * pow(z, q) * pow(b, p1);
         }
@@ -631,6 +634,7 @@ int or_sources(const or_model *m, uint32_t seed, uint32_t iteration,
 typedef struct {
     double x, mu, e, e0, wl;
     int ix, ic, ig, itype, idist, stat;   /* stat 0 live, 1 flux, 2 dead */
+    int nnewton;                           /* Newton-Raphson Doppler distances taken */
 } or_ptcl;
 
 typedef struct {
@@ -720,6 +724,7 @@ static int or_transport11(const or_model *m, or_ptcl *p, double *vx, or_stream *
     double dvdr = (VXA(ix + 1) - VXA(ix)) / DX(ix);
     double vhelp = VXA(ix) - dvdr * XA(ix);
     if (fabs(*vx - vhelp) / PC_C >= 0.1) {
+        p->nnewton++;
         const double epstol = 1e-3;
         int it = 0;
         double shelp = db;
@@ -1284,6 +1289,7 @@ int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result
             ct[0]++;
             p.stat = 0;
             p.idist = 0;
+            p.nnewton = 0;
             int64_t istep = 0;
             int perr = 0;
             while (p.stat == 0) {
@@ -1319,6 +1325,7 @@ int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result
                 }
             }
             if (istep > ct[3]) ct[3] = istep;
+            ct[9] += p.nnewton;
             if (p.stat == 2) {
                 ct[6]++;
                 tot[4] += p.e;

@@ -53,7 +53,8 @@ typedef struct {
     double *luminos, *lumdev;   /* ng (flux grid == group grid) */
     int *lumnum;
     double tot[6];    /* eout, evelo, sflux, estart(lab, after the IMC lab transform), edead, esrc(comoving) */
-    int64_t ct[10];   /* nactive, nstepimc, nstepddmc, nstepmax, nmethswap, nflux, ndead, nerr, nwarn, 0 */
+    int64_t ct[10];   /* nactive, nstepimc, nstepddmc, nstepmax, nmethswap, nflux, ndead, nerr, nwarn,
+                        Newton-Raphson Doppler distances (transport11.f90:159-199) */
     /* optional per-particle final state (NULL to skip) */
     uint8_t *pstat;   /* 1 flux, 2 dead */
     double *px, *pmu, *pwl, *pe;

@@ -220,7 +220,8 @@ class Model:
         ct = list(r.ct)
         out.update(eout=tot[0], evelo=tot[1], sflux=tot[2], estart=tot[3], edead=tot[4],
                    esrc=tot[5], nactive=ct[0], nstepimc=ct[1], nstepddmc=ct[2], nstepmax=ct[3],
-                   nmethswap=ct[4], nflux=ct[5], ndead=ct[6], nerr=ct[7], nwarn=ct[8], rc=rc)
+                   nmethswap=ct[4], nflux=ct[5], ndead=ct[6], nerr=ct[7], nwarn=ct[8],
+                   nnewton=ct[9], rc=rc)
         if per_particle:
             out["particles"] = pp
         return out

@@ -172,6 +172,14 @@ class RefSession:
     def emitgroup(self, r, ic):
         return self.lib.slref_emitgroup(_d(r), _i(ic))
 
+    def sourcenumbers(self, emit, ns):
+        """grd_nvol and src_nnew of the reference's sourcenumbers for grd_emit = emit"""
+        e = np.ascontiguousarray(emit, np.float64)
+        nvol = np.zeros(self.nc, np.int32)
+        nnew = _i()
+        self.lib.slref_sourcenumbers(_i(self.nc), _p(e), _i(int(ns)), _p(nvol), ctypes.byref(nnew))
+        return nvol, nnew.value
+
     def binsrch(self, x, arr, widerange):
         a = np.ascontiguousarray(arr, np.float64)
         return self.lib.slref_binsrch(_d(x), _p(a), _i(len(a)), _i(int(widerange)))

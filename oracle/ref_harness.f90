@@ -297,6 +297,22 @@ contains
     emit = grd_emit
   end subroutine
 
+  !-- sourcenumbers (superlite.f90:182-183) for a given grd_emit and particle count, one rank
+  !   with every slot vacant; returns grd_nvol and src_nnew
+  subroutine slref_sourcenumbers(nc, emit, ns, nvol, nnew) bind(C, name="slref_sourcenumbers")
+    use gridmod, only: grd_emit, grd_nvol
+    use sourcemod, only: src_ns, src_nvacantall, src_nnew
+    integer(c_int), value :: nc, ns
+    real(c_double), intent(in) :: emit(nc)
+    integer(c_int), intent(out) :: nvol(nc), nnew
+    grd_emit = emit
+    src_ns = ns
+    src_nvacantall(1) = int(ns, 8)
+    call sourcenumbers
+    nvol = grd_nvol
+    nnew = src_nnew
+  end subroutine
+
   function slref_emitgroup(r, ic) result(ig) bind(C, name="slref_emitgroup")
     use gridmod, only: emitgroup
     real(c_double), value :: r

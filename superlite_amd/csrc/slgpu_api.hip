@@ -442,18 +442,17 @@ int slgpu_set_ff_table(slgpu_ctx *c, const double *gff)
     return 0;
 }
 
-int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *out_cap64,
-                        float *out_cap32, double *out_sig, double *out_capgrey,
-                        double *out_capross, float *out_emiss)
+}  // extern "C"
+
+namespace {
+
+// The opacity build (O2-O8) with the per-call staging of the ion populations oc and ginv
+// (2 * nion * nc doubles) at `stage`.
+int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *out_cap64,
+                       float *out_cap32, double *out_sig, double *out_capgrey,
+                       double *out_capross, float *out_emiss, double *stage)
 {
-    if (!c || !gas || !gas->temp || !gas->natom || !gas->vol || !gas->mass || !gas->nelec ||
-        !gas->natomfr || (c->nion > 0 && (!gas->oc || !gas->ginv)))
-        return fail(SLGPU_EINVAL, "null gas state");
-    if (!(c->have_groups && c->have_lines && c->have_bf && c->have_ff && c->have_ions))
-        return fail(SLGPU_EORDER, "groups, lines, ions and bf/ff tables must be set first");
     const int nc = c->nx, ng = c->ng, nion = c->nion;
-    for (int i = 0; i < nc; ++i)
-        if (!(gas->temp[i] > 0.0)) return fail(SLGPU_EINVAL, "gas_temp must be positive (gas_update.f:71-72)");
     // present elements (ff adds exactly zero for the others, physical_opacity.f:240-241)
     int nel = 0, elz[30];
     for (int z = 1; z <= 30; ++z) {
@@ -465,11 +464,7 @@ int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *o
     PhaseTimer tm(c, 0);
     // per-call host inputs (small: nc and nion*nc doubles)
     double *dn = c->d.natom_in, *dv = c->d.vol_in, *de = c->d.nelec_in;   // kept for the backward
-    double *doc = nullptr, *dgi = nullptr;
-    int rc = 0;
-    rc |= dalloc(c, &doc, (size_t)std::max(nion, 1) * nc);
-    rc |= dalloc(c, &dgi, (size_t)std::max(nion, 1) * nc);
-    if (rc) return rc;
+    double *doc = stage, *dgi = stage + (size_t)std::max(nion, 1) * nc;
     HIPCHK(hipMemcpyAsync(c->d.temp, gas->temp, nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d.mass, gas->mass, nc * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d.natomfr, gas->natomfr, (size_t)nc * 30 * 8, hipMemcpyHostToDevice, c->stream));
@@ -494,16 +489,47 @@ int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *o
     if (out_sig) HIPCHK(hipMemcpyAsync(out_sig, c->d.sig, nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_capgrey) HIPCHK(hipMemcpyAsync(out_capgrey, c->d.capgrey, nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_capross) HIPCHK(hipMemcpyAsync(out_capross, c->d.capross, nc * 8, hipMemcpyDeviceToHost, c->stream));
-    rc = tm.stop();
+    int rc = tm.stop();
     float kms = 0.f;
     hipEventElapsedTime(&kms, c->ev[10], c->ev[11]);
     c->t_ms[5] += kms;
-    dfree(c, doc); dfree(c, dgi);
     c->opac_flags = flags;
     if (rc) return rc;
     c->have_opacity = true;
     c->have_prep = false;
     return 0;
+}
+
+int check_opacity_inputs(slgpu_ctx *c, const slgpu_gas *gas)
+{
+    if (!c || !gas || !gas->temp || !gas->natom || !gas->vol || !gas->mass || !gas->nelec ||
+        !gas->natomfr || (c->nion > 0 && (!gas->oc || !gas->ginv)))
+        return fail(SLGPU_EINVAL, "null gas state");
+    if (!(c->have_groups && c->have_lines && c->have_bf && c->have_ff && c->have_ions))
+        return fail(SLGPU_EORDER, "groups, lines, ions and bf/ff tables must be set first");
+    for (int i = 0; i < c->nx; ++i)
+        if (!(gas->temp[i] > 0.0)) return fail(SLGPU_EINVAL, "gas_temp must be positive (gas_update.f:71-72)");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *out_cap64,
+                        float *out_cap32, double *out_sig, double *out_capgrey,
+                        double *out_capross, float *out_emiss)
+{
+    int rc = check_opacity_inputs(c, gas);
+    if (rc) return rc;
+    double *stage = nullptr;
+    rc = dalloc(c, &stage, (size_t)2 * std::max(c->nion, 1) * c->nx);
+    if (rc) return rc;
+    rc = opacity_build_impl(c, gas, flags, out_cap64, out_cap32, out_sig, out_capgrey,
+                            out_capross, out_emiss, stage);
+    hipStreamSynchronize(c->stream);
+    dfree(c, stage);
+    return rc;
 }
 
 int slgpu_ddmc_prep(slgpu_ctx *c, const double *radtemp, double lbol, double *out_opaclump,

@@ -16,6 +16,9 @@ namespace slgpu {
 
 #define PH1(p, s, z, e) ((double)ph1[(p - 1) + 6 * ((s - 1) + 7 * ((z - 1) + 30 * (e - 1)))])
 #define PH2(p, z, e) ((double)ph2[(p - 1) + 7 * ((z - 1) + 30 * (e - 1))])
+// bf_ph1(6,..)**2, bf_ph2(5,..)**2 and bf_ph2(7,..)**2 are single-precision expressions in
+// bfxsmod.f:141,148,149: the square is rounded to FP32 before it joins the FP64 sum
+#define SQ32(v) ((double)((float)(v) * (float)(v)))
 
 // bfxsmod.f:65-153: Verner 1995/96 partial photo-ionisation cross sections [Mb]
 __device__ double bfxs_dev(int nz, int ne, double e, const float *__restrict__ ph1,
@@ -46,15 +49,15 @@ __device__ double bfxs_dev(int nz, int ne, double e, const float *__restrict__ p
             double p1 = -PH1(5, is, nz, ne);
             double y = e / PH1(2, is, nz, ne);
             double q = -0.5 * p1 - ll[is - 1] - 5.5;
-            double a = PH1(3, is, nz, ne) * ((y - 1.0) * (y - 1.0) + PH1(6, is, nz, ne) * PH1(6, is, nz, ne));
+            double a = PH1(3, is, nz, ne) * ((y - 1.0) * (y - 1.0) + SQ32(PH1(6, is, nz, ne)));
             double b = sqrt(y / PH1(4, is, nz, ne)) + 1.0;
             xs = xs + a * pow(y, q) * pow(b, p1);
         } else {
             double p1 = -PH2(4, nz, ne);
             double q = -0.5 * p1 - 5.5;
             double x = e / PH2(1, nz, ne) - PH2(6, nz, ne);
-            double z = sqrt(x * x + PH2(7, nz, ne) * PH2(7, nz, ne));
-            double a = PH2(2, nz, ne) * ((x - 1.0) * (x - 1.0) + PH2(5, nz, ne) * PH2(5, nz, ne));
+            double z = sqrt(x * x + SQ32(PH2(7, nz, ne)));
+            double a = PH2(2, nz, ne) * ((x - 1.0) * (x - 1.0) + SQ32(PH2(5, nz, ne)));
             double b = 1.0 + sqrt(z / PH2(3, nz, ne));
             xs = xs + a * pow(z, q) * pow(b, p1);
         }

@@ -166,6 +166,11 @@ module slgpu_iface
       import :: c_ptr
       type(c_ptr), value :: ctx
     end function
+
+    integer(c_int) function slgpu_peak_rates(exp32, exp64) bind(C, name="slgpu_peak_rates")
+      import :: c_int, c_double
+      real(c_double), intent(out) :: exp32, exp64
+    end function
   end interface
 
 contains

@@ -28,7 +28,7 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_set_groups", "slgpu_set_lines", "slgpu_set_bf_tables", "slgpu_set_ff_table",
            "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
-           "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream")
+           "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates")
 
 
 class SlgpuError(RuntimeError):
@@ -78,6 +78,7 @@ def lib():
         L.slgpu_timers.argtypes = [_vp, _vp, _i, _i]
         L.slgpu_stream.restype = _vp
         L.slgpu_stream.argtypes = [_vp]
+        L.slgpu_peak_rates.argtypes = [_vp, _vp]
         _lib = L
     return _lib
 
@@ -239,6 +240,16 @@ class Context:
         self._check(self.L.slgpu_timers(self.h, _p(ms), 8, int(reset)))
         return dict(opacity=ms[0], prep=ms[1], sources=ms[2], transport=ms[3], fluxtally=ms[4],
                     opacity_kernel=ms[5], transport_kernel=ms[6], opacity_vjp=ms[7])
+
+
+def peak_rates():
+    """Measured exponential throughput of the device: (FP32 v_exp_f32 /s, FP64 exp() /s)."""
+    L = lib()
+    a, b = _d(), _d()
+    rc = L.slgpu_peak_rates(ctypes.byref(a), ctypes.byref(b))
+    if rc != 0:
+        raise SlgpuError(rc, "slgpu_peak_rates failed")
+    return a.value, b.value
 
 
 def setup_workload(w, st, wl, lines, tables, precision=SLGPU_FP64, seed=1, npartmax=None,

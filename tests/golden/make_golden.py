@@ -61,9 +61,19 @@ def main():
     ap.add_argument("--mc-iter", type=int, default=16)
     ap.add_argument("--mc-c2", action="store_true",
                     help="only the c2 reference Monte Carlo (2^10 particles, 8 iterations)")
+    ap.add_argument("--sourcenumbers", action="store_true",
+                    help="only the reference's sourcenumbers split (ref_sourcenumbers.npz)")
+    ap.add_argument("--bfxs", action="store_true",
+                    help="only the reference's Verner cross sections (ref_bfxs.npz)")
     args = ap.parse_args()
     if args.mc_c2:
         mc_case("c2", 1.0, 10, 8, workload="c2")
+        return
+    if args.sourcenumbers:
+        sourcenumbers_cases()
+        return
+    if args.bfxs:
+        bfxs_cases()
         return
 
     s0 = RefSession()
@@ -98,6 +108,8 @@ def main():
         print("state", name, "ions", len(st["ion_iz"]), flush=True)
 
     edge_case()
+    sourcenumbers_cases()
+    bfxs_cases()
 
     if args.mc:
         for case, scale in (("c1", 1.0), ("c1mixed", 1e-5)):
@@ -147,6 +159,102 @@ def edge_case():
                         **out)
     s.close()
     print("edge case done", flush=True)
+
+
+def bfxs_cases():
+    """The reference's own Verner partial cross sections bfxs(nz, ne, E) in Mb
+    (bfxsmod.f:65-153): every species nz = 1..30, ne = 1..nz on a log grid of photon
+    energies plus each shell threshold and its neighbouring doubles; and the values at the
+    left group edges of workload c1 and of the edge case for their ion lists, the exact
+    cross sections the bound-free sum of physical_opacity.f:177-189 adds."""
+    s = RefSession()
+    ph1, ph2, _ = s.tables()
+    grid = np.geomspace(1.0, 1e5, 48)
+    nz_l, ne_l, e_l, xs_l = [], [], [], []
+    for nz in range(1, 31):
+        for ne in range(1, nz + 1):
+            th = [float(ph1[ne - 1, nz - 1, is_, 0]) for is_ in range(7)]
+            th = [t for t in th if t > 0.0]
+            es = list(grid)
+            for t in th:
+                es += [np.nextafter(t, 0.0), t, np.nextafter(t, np.inf), t * 1.5]
+            for e in es:
+                nz_l.append(nz); ne_l.append(ne); e_l.append(e)
+                xs_l.append(s.bfxs(nz, ne, float(e)))
+    out = dict(nz=np.array(nz_l, np.int16), ne=np.array(ne_l, np.int16),
+               e=np.array(e_l), xs=np.array(xs_l))
+    hc_ev = 6.6260693e-27 * 2.997924562e10 / 1.60217653e-12
+    for name, wl, iz, ii in (("c1", synth.group_grid(synth.WORKLOADS["c1"]),
+                              *synth.ion_list(synth.WORKLOADS["c1"], s.ni())),
+                             ("edge", np.array([2000e-8, 2500e-8, 3100e-8, 3900e-8, 4800e-8, 6000e-8]),
+                              np.array([2, 8, 8, 26]), np.array([1, 1, 2, 2]))):
+        en = (6.6260693e-27 * 2.997924562e10) / (1.60217653e-12 * wl[:-1])   # physical_opacity.f:179
+        tab = np.array([[s.bfxs(int(z), int(z - i + 1), float(e)) for z, i in zip(iz, ii)]
+                        for e in en])
+        out[f"{name}_group_xs"] = tab
+        out[f"{name}_en"] = en
+    s.close()
+    np.savez_compressed(os.path.join(OUT, "ref_bfxs.npz"), **out)
+    print("bfxs cases", len(xs_l), flush=True)
+
+
+def _first_pass_total(emit, ns):
+    """sum of grd_nvol after the proportional pass of sourcenumbers.f90:58-69, before the
+    exact-total fix-up: labels which cases exercise the fix-up loop (:73-89) and its sign"""
+    etot = float(np.sum(emit))
+    einv = 1.0 / etot
+    invn = 1.0 / ns
+    edone, ndone = 0.0, 0
+    for en in emit:
+        if en == 0.0:
+            continue
+        n = max(1, int(en * ns * einv))
+        if edone * einv > ndone * invn:
+            n += 1
+        edone += en
+        ndone += n
+    return ndone
+
+
+def sourcenumbers_cases():
+    """The reference's own grd_nvol / src_nnew (SOURCE/sourcenumbers.f90:26-169) for the
+    source energies of every workload and for perturbed, heavy-tailed and partly void
+    emission profiles, at particle counts from the cell count up to 2^20."""
+    rows = []
+    for name in ("c1", "c2", "c5"):
+        w = synth.WORKLOADS[name]
+        st = dict(np.load(os.path.join(OUT, f"state_{name}.npz")))
+        s = RefSession()
+        s.setup(w.nx, w.ng, w.wlmin, w.wlmax, st["xarr"], st["vxarr"], n2s=4, nomp=1)
+        from oracle import pyoracle as po
+        emit, _ = po.sourceenergy(st["rcell"], st["temp"], w.lbol)
+        rng = np.random.default_rng(1004)
+        heavy = emit.copy()
+        heavy[rng.integers(0, w.nx, 3)] *= 1e6
+        void = emit * (rng.random(w.nx) > 0.3)
+        void[0] = emit[0]
+        profiles = {"emit": emit, "perturbed": emit * rng.uniform(0.5, 2.0, w.nx),
+                    "heavy": heavy, "void": void}
+        for pname, e in profiles.items():
+            nact = int(np.count_nonzero(e))
+            for ns in (1000, 4096, 12345, 1 << 14, 1 << 20, nact, nact + 1, nact + 7, 3 * nact - 1):
+                if ns < nact:
+                    continue
+                nvol, nnew = s.sourcenumbers(e, ns)
+                rows.append(dict(workload=name, profile=pname, ns=ns, emit=e.copy(),
+                                 nvol=nvol.copy(), nnew=nnew, first=_first_pass_total(e, ns)))
+        s.close()
+    nmax = max(len(r["emit"]) for r in rows)
+    pad = lambda a, dt: np.array([np.pad(r[a], (0, nmax - len(r[a]))) for r in rows], dt)
+    np.savez_compressed(os.path.join(OUT, "ref_sourcenumbers.npz"),
+                        nc=np.array([len(r["emit"]) for r in rows], np.int32),
+                        ns=np.array([r["ns"] for r in rows], np.int64),
+                        emit=pad("emit", np.float64), nvol=pad("nvol", np.int32),
+                        nnew=np.array([r["nnew"] for r in rows], np.int64),
+                        first_pass=np.array([r["first"] for r in rows], np.int64),
+                        label=np.array([f"{r['workload']}/{r['profile']}" for r in rows]))
+    print("sourcenumbers cases", len(rows), "fix-up up/down:",
+          sum(r["first"] < r["ns"] for r in rows), sum(r["first"] > r["ns"] for r in rows), flush=True)
 
 
 def mc_case(case, scale, n2s, niter, workload="c1"):

@@ -184,12 +184,41 @@ def test_sampler_tables_exact(S, tables, name):
         assert len(bad) == 0, (t, rr[bad[:5]], cc[bad[:5]], a[bad[:5]], ref[bad[:5]])
         assert (a[(t == 4) | (a != 0)] >= 1).all() and (a <= w.ng).all()
     m = oracle_model(st, w, wl, g, prep=p)
-    sel = rng.choice(len(rr), 3000, replace=False)
+    sel = rng.choice(len(rr), 20000, replace=False)
     e = ctx.sample_groups(4, rr[sel], cc[sel])
     o = np.array([m.emitgroup(rr[k], cc[k]) for k in sel])
-    say("emitgroup vs oracle agreement", (e == o).mean())
-    assert (e == o).mean() > 0.999
+    bad = np.flatnonzero(e != o)
+    say("emitgroup vs oracle: mismatches", len(bad), "of", len(sel))
+    # every mismatch is a draw within rounding of an edge of the oracle's emitgroup CDF
+    for k in bad:
+        ic = int(cc[sel[k]])
+        r1 = rr[sel[k]] * g["capgrey"][ic - 1]
+        edges = emitgroup_edges(st, w, wl, g, p, ic)
+        gap = np.min(np.abs(edges - r1)) / r1
+        say(f"  r={rr[sel[k]]!r} cell {ic}: gpu {e[k]} oracle {o[k]}, nearest CDF edge {gap:.1e}")
+        assert gap < 1e-12 and abs(int(e[k]) - int(o[k])) == 1
     ctx.close()
+
+
+def emitgroup_edges(st, w, wl, g, p, ic):
+    """every value r*capgrey is compared against in emitgroup.f:23-51 for cell ic: the binned
+    emission probabilities and the in-bin running sums of specintv*cap from each bin's start"""
+    spec = po.specintv(1.0 / st["temp"][ic - 1], 1.0 / wl)
+    cap = g["cap"][ic - 1].astype(np.float64)
+    ep = p["emitprob"][ic - 1]
+    nep, nepg = int(st["nep"]), int(st["nepg"])
+    edges = [ep]
+    for iep in range(nep):
+        g0, g1 = iep * nepg, min(iep * nepg + nepg - 1, w.ng)
+        acc = 0.0 if iep == 0 else ep[iep - 1]
+        run = []
+        for ig in range(g0, g1 - 1 + 1):
+            if ig >= w.ng:
+                break
+            acc = acc + spec[ig] * cap[ig]
+            run.append(acc)
+        edges.append(np.array(run))
+    return np.concatenate(edges)
 
 
 def test_precision_form_c1(S, tables):
@@ -373,8 +402,37 @@ def test_sources_c1_per_particle(S, tables):
     np.testing.assert_allclose(pp["mu"], o["mu"], rtol=1e-13, atol=1e-15)
     np.testing.assert_allclose(pp["e"], o["e"], rtol=1e-14)
     same_wl = np.isclose(pp["wl"], o["wl"], rtol=1e-13)
-    assert same_wl.mean() > 0.9999, same_wl.mean()
+    # a different wavelength means a different group; each such particle's group draw lies
+    # within rounding of an edge of the oracle's cumulative source spectrum
+    bad = np.flatnonzero(~same_wl)
+    say("source wavelength mismatches", len(bad), "of", nnew)
+    pstart = np.concatenate([[0], np.cumsum(nvol)])
+    for q in bad:
+        i = int(np.searchsorted(pstart, q, side="right"))         # owning cell, 1-based
+        prob = po.specintv(1.0 / st["temp"][i - 1], 1.0 / wl)
+        prob = prob / prob.sum()
+        edges = np.cumsum(prob)
+        r = philox_draw0(5, 3, int(q))
+        gap = np.min(np.abs(edges - r)) / r
+        gg = np.searchsorted(wl, [pp["wl"][q], o["wl"][q]])
+        say(f"  particle {q} cell {i}: groups {gg}, draw {r!r}, nearest CDF edge {gap:.1e}")
+        assert gap < 1e-12 and abs(int(gg[0]) - int(gg[1])) == 1
     ctx.close()
+
+
+def philox_draw0(seed, iteration, p):
+    """draw 0 of source particle p: Philox-4x32-10 block (p, 0, 0, 0) under key
+    (seed, iteration), words 0 and 1 as a 53-bit uniform"""
+    import ctypes
+
+    class U4(ctypes.Structure):
+        _fields_ = [("v", ctypes.c_uint32 * 4)]
+    f = po.lib().or_philox_block
+    f.restype = U4
+    f.argtypes = [U4, ctypes.c_uint32, ctypes.c_uint32]
+    out = f(U4((ctypes.c_uint32 * 4)(p, 0, 0, 0)), seed, iteration)
+    a, b = out.v[0], out.v[1]
+    return (float(((a >> 5) << 26) | (b >> 6)) + 0.5) * 2.0 ** -53
 
 
 def run_both(S, tables, n2s, seed, it, scale=1.0, max_steps=0):

@@ -87,6 +87,116 @@ def test_bfxs_hydrogen_threshold(tables):
     assert -3.6 < np.log2(b / a) < -2.9
 
 
+def test_bfxs_vs_reference(golden, tables):
+    """or_bfxs against the reference's own bfxs (bfxsmod.f:65-153) for every species
+    nz = 1..30, ne = 1..nz: a log grid of photon energies, each shell threshold and the
+    doubles beside it (the threshold tests of :116, :132-133 decide on those)."""
+    ph1, ph2, _ = tables
+    g = golden("ref_bfxs.npz")
+    got = np.array([po.bfxs(int(z), int(n), float(e), ph1, ph2)
+                    for z, n, e in zip(g["nz"], g["ne"], g["e"])])
+    zero = g["xs"] == 0.0
+    assert np.array_equal(got == 0.0, zero)
+    np.testing.assert_allclose(got[~zero], g["xs"][~zero], rtol=1e-13)
+
+
+def _ld_bf(e, xs, mass):
+    """sum over ions of xs(Mb) * 1e-18 * oc in extended precision (physical_opacity.f:177-189)"""
+    L = np.longdouble
+    oc = e["oc"].astype(L)                                   # (nc, nion)
+    cap = (xs.astype(L) * L(1e-18))[None, :, :] * oc[:, None, :]
+    out = cap.sum(-1)
+    out[mass <= 0] = 0
+    return out
+
+
+def test_bf_cap64_longdouble(golden, tables, c1):
+    """The oracle's FP64 bound-free opacity (flags: no bb, no ff) against an independent
+    extended-precision sum of the reference's own cross sections at each group's left
+    edge, for workload c1 (50 cells x 1000 groups x 18 ions) and the edge case, to 1e-12."""
+    g = golden("ref_bfxs.npz")
+    w, st, wl, lines = c1
+    o = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"],
+                   st["ion_iz"], st["ion_ii"], st["oc"], st["ginv"], lines, tables, flags=5)
+    ref = _ld_bf(st, g["c1_group_xs"], st["mass"])
+    m = ref > 0
+    assert m.mean() > 0.1              # bf reaches the groups above the ions' thresholds
+    np.testing.assert_allclose(o["cap64"][m], ref[m].astype(float), rtol=1e-12)
+    assert np.all(o["cap64"][~m] == 0)
+    e = golden("ref_edge_opacity.npz")
+    o = po.opacity(e["wl"], e["temp"], e["natom"], e["vol"], e["mass"], e["nelec"], e["natomfr"],
+                   e["ion_iz"], e["ion_ii"], e["oc"], e["ginv"],
+                   (e["wl0"], e["gxs"], e["chilw"], e["liz"], e["lii"]), tables, flags=5)
+    ref = _ld_bf(e, g["edge_group_xs"], e["mass"])
+    m = ref > 0
+    np.testing.assert_allclose(o["cap64"][m], ref[m].astype(float), rtol=1e-12)
+    assert np.all(o["cap64"][~m] == 0)
+
+
+def _ld_ff(wl, temp, natom, vol, nelec, natomfr, mass, gff):
+    """free-free opacity (physical_opacity.f:199-247) in extended precision: Gaunt factor
+    bilinear in (log10 u, log10 gamma^2) on the 81 x 41 table, clamped indices and
+    unclamped weights, max(gff, 1) past the gamma^2 end; the table cell is chosen from the
+    FP64 coordinates, as the reference's floor() on its FP64 values chooses it."""
+    L = np.longdouble
+    h, c, kb = L("6.6260693e-27"), L("2.997924562e10"), L("1.3806505e-16")
+    e_, me, ryd = L("4.80325e-10"), L("9.1093826e-28"), L("1.0973731534e5")
+    pi = L("3.1415926535897932385")
+    c1 = 4 * e_ ** 6 / (3 * h * me * c ** 4) * np.sqrt(2 * pi / (3 * me * h * c))
+    T = np.asarray(gff, np.float64).reshape(41, 81)          # T[igg-1, iu-1] = ff_gff(iu, igg)
+    hckt64 = (6.6260693e-27 * 2.997924562e10) / (1.3806505e-16 * temp)
+    hckt = h * c / (kb * temp.astype(L))
+    wle = wl[:-1]
+    u64 = hckt64[:, None] * (1.0 / wle)[None, :]
+    ru64 = 10.0 * (np.log10(u64) + 4.0) + 1.0
+    iu = np.clip(np.floor(ru64).astype(int), 1, 80)
+    u = hckt[:, None] / wle.astype(L)[None, :]
+    du = (10 * (np.log10(u) + 4) + 1) - iu
+    help_ = c1 * np.sqrt(hckt)[:, None] * (1 - np.exp(-u)) * wle.astype(L)[None, :] ** 3 * \
+        ((natom.astype(L) / vol.astype(L)) ** 2 * nelec.astype(L))[:, None]
+    cap = np.zeros(u.shape, L)
+    for iz in range(1, 31):
+        fr = natomfr[iz - 1].astype(L)
+        if not np.any(fr != 0):
+            continue
+        gg64 = (iz * iz) * 1.0973731534e5 * hckt64
+        rgg64 = 5.0 * (np.log10(gg64) + 4.0) + 1.0
+        igg = np.clip(np.floor(rgg64).astype(int), 1, 40)
+        dgg = (5 * (np.log10(L(iz * iz) * ryd * hckt) + 4) + 1) - igg
+        t00 = T[(igg - 1)[:, None], iu - 1]
+        t10 = T[(igg - 1)[:, None], iu]
+        t01 = T[igg[:, None], iu - 1]
+        t11 = T[igg[:, None], iu]
+        dg = dgg[:, None]
+        g = (1 - du) * (1 - dg) * t00 + du * (1 - dg) * t10 + (1 - du) * dg * t01 + du * dg * t11
+        g = np.where((rgg64 > 41)[:, None], np.maximum(g, L(1)), g)
+        cap += help_ * g * L(iz * iz) * fr[:, None]
+    cap[mass <= 0] = 0
+    return cap
+
+
+def test_ff_cap64_longdouble(golden, tables, c1):
+    """The oracle's FP64 free-free opacity (flags: no bb, no bf) against an independent
+    extended-precision evaluation on the reference's own g_ff table, for workload c1
+    (50 cells x 1000 groups x 4 elements) and the edge case, to 1e-12."""
+    w, st, wl, lines = c1
+    gff = tables[2]
+    o = po.opacity(wl, st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"],
+                   st["ion_iz"], st["ion_ii"], st["oc"], st["ginv"], lines, tables, flags=3)
+    ref = _ld_ff(wl, st["temp"], st["natom"], st["vol"], st["nelec"], st["natomfr"], st["mass"], gff)
+    m = ref > 0
+    assert m.all()
+    np.testing.assert_allclose(o["cap64"][m], ref[m].astype(float), rtol=1e-12)
+    e = golden("ref_edge_opacity.npz")
+    o = po.opacity(e["wl"], e["temp"], e["natom"], e["vol"], e["mass"], e["nelec"], e["natomfr"],
+                   e["ion_iz"], e["ion_ii"], e["oc"], e["ginv"],
+                   (e["wl0"], e["gxs"], e["chilw"], e["liz"], e["lii"]), tables, flags=3)
+    ref = _ld_ff(e["wl"], e["temp"], e["natom"], e["vol"], e["nelec"], e["natomfr"], e["mass"], gff)
+    m = ref > 0
+    np.testing.assert_allclose(o["cap64"][m], ref[m].astype(float), rtol=1e-12)
+    assert np.all(o["cap64"][~m] == 0)
+
+
 # --- opacity vs SuperLite's own output -------------------------------------------
 def test_opacity_c1_vs_reference(c1, tables, golden):
     w, st, wl, lines = c1
@@ -185,6 +295,21 @@ def test_emitgroup_vs_reference(c1, golden):
     assert np.array_equal(got, g["eg_ig"])
     # H7: with ng=1000, nep=32, nepg=31, groups above 992 are never re-emitted
     assert got.max() <= 992
+
+
+def test_sourcenumbers_vs_reference(golden):
+    """or_sourcenumbers equals the reference's own grd_nvol exactly (integer split of
+    SOURCE/sourcenumbers.f90:58-89) for the source energies of c1, c2 and c5 and for
+    perturbed, heavy-tailed and partly void profiles, from the cell count up to 2^20
+    particles; the fixture includes cases whose proportional pass overshoots, so the
+    exact-total fix-up loop (:73-89) and its cell order are exercised."""
+    g = golden("ref_sourcenumbers.npz")
+    assert np.count_nonzero(g["first_pass"] > g["ns"]) >= 20
+    for k in range(len(g["ns"])):
+        nc = int(g["nc"][k])
+        got = po.sourcenumbers(g["emit"][k, :nc], int(g["ns"][k]))
+        assert np.array_equal(got, g["nvol"][k, :nc]), (str(g["label"][k]), int(g["ns"][k]))
+        assert got.sum() == g["nnew"][k] == g["ns"][k]
 
 
 def test_sourcenumbers_invariants(c1, golden):
