@@ -62,6 +62,10 @@ typedef struct {
     uint32_t seed;         /* key of the counter-based per-particle RNG (replaces randommod.f) */
     int64_t npartmax;      /* prt_npartmax (particlemod.f90:175) */
     int64_t max_steps;     /* per-particle step cap, a particle beyond it dies with error 200 (0: 2^26) */
+    int ampfact;           /* 1: amplification factor on, i.e. in_trn_noamp = .false.
+                              (inputparmod.f:57, transport11.f90:410-421); 0: the default */
+    int thm_aniso;         /* 1: in_thm_aniso, the experimental anisotropic Thomson angle
+                              (inputparmod.f:58, transport11.f90:258-285); 0: the default */
 } slgpu_config;
 
 /* Host gas state consumed by the opacity build (gasmod.f:20-34 and ion_grndlev,
@@ -126,12 +130,14 @@ int slgpu_sources(slgpu_ctx *ctx, int64_t nstot, uint32_t iteration,
                   int *out_nvol /* nc, may be NULL */, int64_t *out_nnew);
 
 /* particle_advance.f90:65-325 with TRANSPORT1/transport11.f90 and diffusion11.f90.
- * out_totals = [tot_eout, tot_evelo, tot_sflux, e_start, e_dead, e_source]
+ * out_eamp = grd_eamp (nc; zero unless cfg.ampfact, particle_advance.f90:216-217).
+ * out_totals = [tot_eout, tot_evelo, tot_sflux, e_start, e_dead, e_source, sum(grd_eamp)];
+ * e_start = tot_eout + tot_evelo + e_dead + sum(grd_eamp) up to round-off
  * out_counters = [npactive, npstepimc, npstepddmc, npstepmax, npmethswap, npflux,
  *                 ndead, nerror, nwarning, 0] (countersmod.f:12-20) */
 int slgpu_transport(slgpu_ctx *ctx, uint32_t iteration, double *out_tally /* nc */,
-                    double *out_jrad /* ng*nc */, double *out_totals /* 6 */,
-                    int64_t *out_counters /* 10 */);
+                    double *out_jrad /* ng*nc */, double *out_eamp /* nc */,
+                    double *out_totals /* 7 */, int64_t *out_counters /* 10 */);
 
 /* fluxtally.f:25-54: flx_luminos, flx_lumdev, flx_lumnum over the group grid */
 int slgpu_fluxtally(slgpu_ctx *ctx, double *out_luminos, double *out_lumdev,

@@ -689,16 +689,35 @@ static inline void vinterp(const or_model *m, int ix, double x, double *vx)
     *vx = (VXA(ix) * (h2 - x) + VXA(ix + 1) * (x - h1)) / DX(ix);
 }
 
-/* TRANSPORT1/transport11.f90:86-489 — one IMC event (trn_noampfact = .true.) */
+/* transport11.f90:259-285: the experimental anisotropic Thomson cosine.  The second cube
+ * root has a negative base for r1 below ~0.957, where the real power is NaN; every range
+ * test then fails and the isotropic cosine is taken, as in the reference. */
+static double or_thomson_aniso_mu(double r1)
+{
+    const double q = sqrt(1.0 + 16.0 * ((r1 - 0.5) * (r1 - 0.5)));
+    const double mutemp = pow(4.0 * r1 - 2.0 + q, 1.0 / 3.0) - pow(q - (4.0 - 2.0 * r1), 1.0 / 3.0);
+    const double muhelp = mutemp;
+    const double B = 2 * mutemp * muhelp;
+    const double C = mutemp * mutemp + muhelp * muhelp - 1;
+    const double munew1 = (-B + sqrt(B * B - 4 * C)) / 2;
+    const double munew2 = (-B - sqrt(B * B - 4 * C)) / 2;
+    const int ok1 = munew1 <= 1 && munew1 >= -1, ok2 = munew2 <= 1 && munew2 >= -1;
+    if (ok1 && ok2) return r1 < 0.5 ? munew1 : munew2;
+    if (ok1) return munew1;
+    if (ok2) return munew2;
+    return 1.0 - 2.0 * r1;
+}
+
+/* TRANSPORT1/transport11.f90:86-489 — one IMC event */
 static int or_transport11(const or_model *m, or_ptcl *p, double *vx, or_stream *s,
-                          double *eraddens, double *jrad, double *totevelo)
+                          double *eraddens, double *jrad, double *eamp, double *totevelo)
 {
     const int ng = m->ng, nx = m->nx;
     int ix = p->ix, ic = p->ic, ig = p->ig;
     double x = p->x, mu = p->mu, e = p->e, e0 = p->e0, wl = p->wl;
     double r1, r2, db, dcol, dthm, ddop, d;
     int ixnext;
-    *eraddens = 0.0; *jrad = 0.0;
+    *eraddens = 0.0; *jrad = 0.0; *eamp = 0.0;
     double elabfact = 1.0 - mu * (*vx) * CINV;
     double elabfactold = elabfact;
     double far = 2.0 * fabs(sqrt(XA(ix + 1) * XA(ix + 1) - XA(ix) * XA(ix)));
@@ -790,7 +809,8 @@ static int or_transport11(const or_model *m, or_ptcl *p, double *vx, or_stream *
     /* collisions :256-296 */
     if (d == dthm || d == dcol) {
         r1 = or_draw(s);
-        mu = 1.0 - 2.0 * r1;
+        if (d == dthm && m->thm_aniso) mu = or_thomson_aniso_mu(r1);
+        else mu = 1.0 - 2.0 * r1;
         mu = (mu + (*vx) * CINV) / (1.0 + (*vx) * mu * CINV);
     } else if (d == db) {
         if (mu >= 0.0 && ix == nx) {
@@ -867,6 +887,16 @@ static int or_transport11(const or_model *m, or_ptcl *p, double *vx, or_stream *
             ic = ix;
         } else {
             mu = (mu - (*vx) * CINV) / (1.0 - (*vx) * mu * CINV);
+            if (m->ampfact && mu < 0.0) {
+                /* amplification factor :410-421 */
+                double h = 1.0 / fabs(mu);
+                h = h < 100.0 ? h : 100.0;
+                *totevelo = *totevelo - e * 2.0 * (0.55 * h - 1.25 * fabs(mu)) * (*vx) * CINV;
+                *eamp = e * 2.0 * 0.55 * (h - 2.0 > 0.0 ? h - 2.0 : 0.0) * (*vx) * CINV;
+                h = h < 2.0 ? h : 2.0;
+                e0 = e0 * (1.0 + 2.0 * (0.55 * h - 1.25 * fabs(mu)) * (*vx) * CINV);
+                e = e * (1.0 + 2.0 * (0.55 * h - 1.25 * fabs(mu)) * (*vx) * CINV);
+            }
             double help = (CAP(ig, l) + SIG(l)) * DX(ix - 1);
             help = 4.0 / (3.0 * help + 6.0 * PC_DEXT);
             r1 = or_draw(s);
@@ -1231,17 +1261,19 @@ int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result
     memset(res->luminos, 0, sizeof(double) * ng);
     memset(res->lumdev, 0, sizeof(double) * ng);
     memset(res->lumnum, 0, sizeof(int) * ng);
-    for (int k = 0; k < 6; ++k) res->tot[k] = 0.0;
+    if (res->eamp) memset(res->eamp, 0, sizeof(double) * nc);
+    for (int k = 0; k < 7; ++k) res->tot[k] = 0.0;
     for (int k = 0; k < 10; ++k) res->ct[k] = 0;
     int any_err = 0;
 
 #pragma omp parallel
     {
         double *tally = calloc(nc, sizeof(double));
+        double *eamp = calloc(nc, sizeof(double));
         double *jrad = calloc((size_t)ng * nc, sizeof(double));
         double *lum = calloc(ng, sizeof(double)), *dev = calloc(ng, sizeof(double));
         int64_t *num = calloc(ng, sizeof(int64_t));
-        double tot[6] = {0, 0, 0, 0, 0, 0};
+        double tot[7] = {0, 0, 0, 0, 0, 0, 0};
         int64_t ct[10] = {0};
         or_cache cache;
         memset(&cache, 0, sizeof cache);
@@ -1300,7 +1332,12 @@ int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result
                 int ierr;
                 if (p.itype == 1 || m->puretran) {
                     ct[1]++;
-                    ierr = or_transport11(m, &p, &vx, &st, &erd, &jr, &tot[1]);
+                    double ea;
+                    ierr = or_transport11(m, &p, &vx, &st, &erd, &jr, &ea, &tot[1]);
+                    if (m->ampfact) {                 /* particle_advance.f90:216-217 */
+                        eamp[icold - 1] += ea;
+                        tot[6] += ea;
+                    }
                     if (p.itype != 1) ct[4]++;
                 } else {
                     ct[2]++;
@@ -1347,20 +1384,22 @@ int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result
 #pragma omp critical
         {
             for (int i = 0; i < nc; ++i) res->tally[i] += tally[i];
+            if (res->eamp)
+                for (int i = 0; i < nc; ++i) res->eamp[i] += eamp[i];
             for (size_t k = 0; k < (size_t)ng * nc; ++k) res->jrad[k] += jrad[k];
             for (int g = 0; g < ng; ++g) {
                 res->luminos[g] += lum[g];
                 res->lumdev[g] += dev[g];
                 res->lumnum[g] += (int)num[g];
             }
-            for (int k = 0; k < 6; ++k) res->tot[k] += tot[k];
+            for (int k = 0; k < 7; ++k) res->tot[k] += tot[k];
             for (int k = 0; k < 10; ++k) {
                 if (k == 3) { if (ct[3] > res->ct[3]) res->ct[3] = ct[3]; }
                 else res->ct[k] += ct[k];
             }
             if (lerr) any_err = 1;
         }
-        free(tally); free(jrad); free(lum); free(dev); free(num);
+        free(tally); free(eamp); free(jrad); free(lum); free(dev); free(num);
         free(cache.specarr); free(cache.glumps); free(cache.llumps);
     }
     free(pstart);

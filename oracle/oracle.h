@@ -37,6 +37,8 @@ typedef struct {
     double tauddmc, taulump;
     int puretran;
     int max_steps;                /* per-particle step cap (0 = none) */
+    int ampfact;                  /* !in_trn_noamp: amplification factor (transport11.f90:410-421) */
+    int thm_aniso;                /* in_thm_aniso: anisotropic Thomson angle (transport11.f90:258-285) */
     /* per-iteration state (grd_* after bcast_nonpermanent, mpimod_ser.f:98-112) */
     const float *cap;             /* ng*nc */
     const double *sig, *capgrey, *tempinv;   /* nc */
@@ -50,9 +52,11 @@ typedef struct {
 typedef struct {
     double *tally;    /* nc  grd_tally */
     double *jrad;     /* ng*nc grd_jrad */
+    double *eamp;     /* nc  grd_eamp (particle_advance.f90:216-217), may be NULL */
     double *luminos, *lumdev;   /* ng (flux grid == group grid) */
     int *lumnum;
-    double tot[6];    /* eout, evelo, sflux, estart(lab, after the IMC lab transform), edead, esrc(comoving) */
+    double tot[7];    /* eout, evelo, sflux, estart(lab, after the IMC lab transform), edead, esrc(comoving),
+                         eamp (sum of grd_eamp) */
     int64_t ct[10];   /* nactive, nstepimc, nstepddmc, nstepmax, nmethswap, nflux, ndead, nerr, nwarn,
                         Newton-Raphson Doppler distances (transport11.f90:159-199) */
     /* optional per-particle final state (NULL to skip) */

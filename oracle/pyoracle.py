@@ -70,14 +70,15 @@ class _Model(ctypes.Structure):
     _fields_ = [("nx", _i), ("xarr", _vp), ("vxarr", _vp), ("divv", _vp), ("vol", _vp),
                 ("ng", _i), ("wl", _vp), ("wlinv", _vp), ("nep", _i), ("nepg", _i),
                 ("tauddmc", _d), ("taulump", _d), ("puretran", _i), ("max_steps", _i),
+                ("ampfact", _i), ("thm_aniso", _i),
                 ("cap", _vp), ("sig", _vp), ("capgrey", _vp), ("tempinv", _vp),
                 ("opaclump", _vp), ("emitprob", _vp), ("emit", _vp), ("radtemp", _vp),
                 ("nvol", _vp)]
 
 
 class _Result(ctypes.Structure):
-    _fields_ = [("tally", _vp), ("jrad", _vp), ("luminos", _vp), ("lumdev", _vp),
-                ("lumnum", _vp), ("tot", _d * 6), ("ct", ctypes.c_int64 * 10),
+    _fields_ = [("tally", _vp), ("jrad", _vp), ("eamp", _vp), ("luminos", _vp), ("lumdev", _vp),
+                ("lumnum", _vp), ("tot", _d * 7), ("ct", ctypes.c_int64 * 10),
                 ("pstat", _vp), ("px", _vp), ("pmu", _vp), ("pwl", _vp), ("pe", _vp),
                 ("pig", _vp), ("pix", _vp), ("pnstep", _vp), ("perr", _vp)]
 
@@ -149,7 +150,8 @@ class Model:
 
     def __init__(self, xarr, vxarr, divv, vol, wl, nep, nepg, cap, sig, capgrey, tempinv,
                  radtemp, tauddmc=5.0, taulump=10.0, puretran=False, max_steps=0,
-                 opaclump=None, emitprob=None, emit=None, nvol=None):
+                 opaclump=None, emitprob=None, emit=None, nvol=None, ampfact=False,
+                 thm_aniso=False):
         self.a = dict(xarr=_c(xarr, np.float64), vxarr=_c(vxarr, np.float64),
                       divv=_c(divv, np.float64), vol=_c(vol, np.float64),
                       wl=_c(wl, np.float64), wlinv=_c(1.0 / np.asarray(wl, np.float64), np.float64),
@@ -159,7 +161,8 @@ class Model:
         self.nx = len(self.a["divv"])
         self.ng = len(self.a["wl"]) - 1
         self.nep, self.nepg = nep, nepg
-        self.params = (tauddmc, taulump, int(puretran), int(max_steps))
+        self.params = (tauddmc, taulump, int(puretran), int(max_steps), int(ampfact),
+                       int(thm_aniso))
         for k, v in (("opaclump", opaclump), ("emitprob", emitprob), ("emit", emit)):
             self.a[k] = None if v is None else _c(v, np.float64)
         self.a["nvol"] = None if nvol is None else _c(nvol, np.int32)
@@ -174,7 +177,7 @@ class Model:
         t = self.params
         return _Model(self.nx, _p(a["xarr"]), _p(a["vxarr"]), _p(a["divv"]), _p(a["vol"]),
                       self.ng, _p(a["wl"]), _p(a["wlinv"]), self.nep, self.nepg, t[0], t[1],
-                      t[2], t[3], _p(a["cap"]), _p(a["sig"]), _p(a["capgrey"]),
+                      t[2], t[3], t[4], t[5], _p(a["cap"]), _p(a["sig"]), _p(a["capgrey"]),
                       _p(a["tempinv"]), _p(a["opaclump"]), _p(a["emitprob"]), _p(a["emit"]),
                       _p(a["radtemp"]), _p(a["nvol"]))
 
@@ -204,22 +207,22 @@ class Model:
     def iteration(self, seed, iteration, per_particle=False):
         nc, ng = self.nx, self.ng
         n = int(self.a["nvol"].sum())
-        out = dict(tally=np.zeros(nc), jrad=np.zeros((nc, ng)), luminos=np.zeros(ng),
+        out = dict(tally=np.zeros(nc), jrad=np.zeros((nc, ng)), eamp=np.zeros(nc), luminos=np.zeros(ng),
                    lumdev=np.zeros(ng), lumnum=np.zeros(ng, np.int32))
         pp = {}
         if per_particle:
             pp = dict(stat=np.zeros(n, np.uint8), x=np.zeros(n), mu=np.zeros(n), wl=np.zeros(n),
                       e=np.zeros(n), ig=np.zeros(n, np.int32), ix=np.zeros(n, np.int32),
                       nstep=np.zeros(n, np.int64), err=np.zeros(n, np.int32))
-        r = _Result(_p(out["tally"]), _p(out["jrad"]), _p(out["luminos"]), _p(out["lumdev"]),
-                    _p(out["lumnum"]), (_d * 6)(), (ctypes.c_int64 * 10)(),
+        r = _Result(_p(out["tally"]), _p(out["jrad"]), _p(out["eamp"]), _p(out["luminos"]),
+                    _p(out["lumdev"]), _p(out["lumnum"]), (_d * 7)(), (ctypes.c_int64 * 10)(),
                     *[_p(pp.get(k)) for k in ("stat", "x", "mu", "wl", "e", "ig", "ix", "nstep", "err")])
         s = self.struct()
         rc = lib().or_iteration(ctypes.byref(s), seed, iteration, ctypes.byref(r))
         tot = list(r.tot)
         ct = list(r.ct)
         out.update(eout=tot[0], evelo=tot[1], sflux=tot[2], estart=tot[3], edead=tot[4],
-                   esrc=tot[5], nactive=ct[0], nstepimc=ct[1], nstepddmc=ct[2], nstepmax=ct[3],
+                   esrc=tot[5], eampsum=tot[6], nactive=ct[0], nstepimc=ct[1], nstepddmc=ct[2], nstepmax=ct[3],
                    nmethswap=ct[4], nflux=ct[5], ndead=ct[6], nerr=ct[7], nwarn=ct[8],
                    nnewton=ct[9], rc=rc)
         if per_particle:

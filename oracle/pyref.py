@@ -80,12 +80,14 @@ class RefSession:
 
     # -- setup ------------------------------------------------------------
     def setup(self, nx, ng, wlmin, wlmax, xleft, vxleft, n2s=10, nomp=1,
-              tauddmc=5.0, taulump=10.0, puretran=False, errorfatal=True, lbol=1e43):
+              tauddmc=5.0, taulump=10.0, puretran=False, errorfatal=True, lbol=1e43,
+              ampfact=False, thm_aniso=False):
         xl = np.ascontiguousarray(xleft, np.float64)
         vl = np.ascontiguousarray(vxleft, np.float64)
         self.lib.slref_setup(_i(nx), _i(ng), _d(wlmin), _d(wlmax), _p(xl), _p(vl),
                              _i(n2s), _i(nomp), _d(tauddmc), _d(taulump),
-                             _i(int(puretran)), _i(int(errorfatal)), _d(lbol))
+                             _i(int(puretran)), _i(int(errorfatal)), _d(lbol),
+                             _i(int(ampfact)), _i(int(thm_aniso)))
         self.nc, self.ng = nx, ng
         vol = np.zeros(nx)
         rcell = np.zeros(nx)
@@ -201,14 +203,15 @@ class RefSession:
         nc, ng = self.nc, self.ng
         tally = np.zeros(nc)
         jrad = np.zeros((nc, ng))
+        eamp = np.zeros(nc)
         lum = np.zeros(ng)
         dev = np.zeros(ng)
         num = np.zeros(ng, np.int32)
         tot = np.zeros(4)
         ct = np.zeros(8, np.int64)
-        self.lib.slref_iteration(_i(nc), _i(ng), _p(tally), _p(jrad), _p(lum), _p(dev),
-                                 _p(num), _p(tot), _p(ct))
-        return dict(tally=tally, jrad=jrad, luminos=lum, lumdev=dev, lumnum=num,
+        self.lib.slref_iteration(_i(nc), _i(ng), _p(tally), _p(jrad), _p(eamp), _p(lum),
+                                 _p(dev), _p(num), _p(tot), _p(ct))
+        return dict(tally=tally, jrad=jrad, eamp=eamp, luminos=lum, lumdev=dev, lumnum=num,
                     eout=tot[0], evelo=tot[1], sflux=tot[2], sthermal=tot[3],
                     nnew=int(ct[0]), nactive=int(ct[1]), nstepimc=int(ct[2]),
                     nstepddmc=int(ct[3]), nstepmax=int(ct[4]), nmethswap=int(ct[5]),

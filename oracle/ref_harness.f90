@@ -52,7 +52,7 @@ contains
   !-- grid, groups, flux grid, transport pointers, particle store, rng
   !   (mirrors superlite.f90:87-121 without file input)
   subroutine slref_setup(nx, ng, wlmin, wlmax, xleft, vxleft, n2s, nomp, &
-       tauddmc, taulump, puretran, errorfatal, lbol) bind(C, name="slref_setup")
+       tauddmc, taulump, puretran, errorfatal, lbol, ampfact, thm_aniso) bind(C, name="slref_setup")
     use inputparmod
     use inputstrmod, only: str_xleft, str_yleft, str_zleft, str_vxleft, str_vyleft, &
          str_vzleft, str_idcell, str_nc
@@ -67,7 +67,7 @@ contains
     use particlemod, only: particle_alloc
     use randommod, only: rnd_init, rnd_state, rnd_states
 !$  use omp_lib, only: omp_set_num_threads
-    integer(c_int), value :: nx, ng, n2s, nomp, puretran, errorfatal
+    integer(c_int), value :: nx, ng, n2s, nomp, puretran, errorfatal, ampfact, thm_aniso
     real(c_double), value :: wlmin, wlmax, tauddmc, taulump, lbol
     real(c_double), intent(in) :: xleft(nx+1), vxleft(nx+1)
     integer :: i
@@ -84,6 +84,8 @@ contains
     in_taulump = taulump
     in_puretran = puretran /= 0
     in_trn_errorfatal = errorfatal /= 0
+    in_trn_noamp = ampfact == 0
+    in_thm_aniso = thm_aniso /= 0
     in_L_bol = lbol
     in_flx_ndim = [0, 1, 1]
     call provide_inputpars(1)
@@ -357,7 +359,7 @@ contains
   !-- one Monte Carlo iteration after the prep (superlite.f90:182-219): sources, transport, flux tally.
   !   out_tot = [eout, evelo, sflux, sthermal]; out_ct = [nnew, nactive, nstepimc, nstepddmc,
   !   nstepmax, nmethswap, nflux, 0]
-  subroutine slref_iteration(nc, ng, tally, jrad, luminos, lumdev, lumnum, out_tot, out_ct) &
+  subroutine slref_iteration(nc, ng, tally, jrad, eamp, luminos, lumdev, lumnum, out_tot, out_ct) &
        bind(C, name="slref_iteration")
     use gridmod
     use sourcemod
@@ -367,7 +369,7 @@ contains
     use countersmod
     use timingmod
     integer(c_int), value :: nc, ng
-    real(c_double), intent(out) :: tally(nc), jrad(ng, nc), luminos(ng), lumdev(ng)
+    real(c_double), intent(out) :: tally(nc), jrad(ng, nc), eamp(nc), luminos(ng), lumdev(ng)
     integer(c_int), intent(out) :: lumnum(ng)
     real(c_double), intent(out) :: out_tot(4)
     integer(c_long_long), intent(out) :: out_ct(8)
@@ -388,6 +390,7 @@ contains
     call fluxtally
     tally = grd_tally
     jrad = grd_jrad
+    eamp = grd_eamp
     luminos = flx_luminos(:, 1, 1)
     lumdev = flx_lumdev(:, 1, 1)
     lumnum = flx_lumnum(:, 1, 1)

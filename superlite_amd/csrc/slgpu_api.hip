@@ -130,6 +130,8 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
         return fail(SLGPU_EINVAL, "npartmax must be in 1..2^31");
     if (cfg->precision != SLGPU_FP64 && cfg->precision != SLGPU_MIXED)
         return fail(SLGPU_EINVAL, "precision must be SLGPU_FP64 or SLGPU_MIXED");
+    if ((cfg->ampfact != 0 && cfg->ampfact != 1) || (cfg->thm_aniso != 0 && cfg->thm_aniso != 1))
+        return fail(SLGPU_EINVAL, "ampfact and thm_aniso must be 0 or 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(SLGPU_EHIP, "no HIP device visible");
@@ -147,6 +149,8 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     d.taulump = cfg->taulump;
     d.puretran = cfg->puretran;
     d.precision = cfg->precision;
+    d.ampfact = cfg->ampfact != 0;
+    d.thm_aniso = cfg->thm_aniso != 0;
     d.max_steps = cfg->max_steps > 0 ? cfg->max_steps : ((int64_t)1 << 26);
     d.seed = cfg->seed;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -213,6 +217,7 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.pnstep, np);
     rc |= dalloc(c, &d.pix, np);
     rc |= dalloc(c, &d.tally, nc);
+    rc |= dalloc(c, &d.eamp, nc);
     // grd_jrad gets one atomic per event; on small grids every lane hits a few hot
     // (cell, group) entries, so blocks add into one of R replicas (R from a 256 MiB budget,
     // 1..32) that a reduction folds into replica 0 after the transport
@@ -599,7 +604,7 @@ int slgpu_sources(slgpu_ctx *c, int64_t nstot, uint32_t iteration, int *out_nvol
 }
 
 int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double *out_jrad,
-                    double *out_totals, int64_t *out_counters)
+                    double *out_eamp, double *out_totals, int64_t *out_counters)
 {
     if (!c) return fail(SLGPU_EINVAL, "null context");
     if (!c->have_sources) return fail(SLGPU_EORDER, "slgpu_sources must run first");
@@ -607,6 +612,7 @@ int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double 
     PhaseTimer tm(c, 3);
     // particle_advance.f90:65-67: zero the tallies
     HIPCHK(hipMemsetAsync(c->d.tally, 0, nc * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->d.eamp, 0, nc * 8, c->stream));
     HIPCHK(hipMemsetAsync(c->d.jrad, 0, (size_t)c->d.jrad_copies * nc * ng * 8, c->stream));
     HIPCHK(hipMemsetAsync(c->d.tot, 0, 8 * 8, c->stream));
     HIPCHK(hipMemsetAsync(c->d.ct, 0, 10 * 8, c->stream));
@@ -621,6 +627,7 @@ int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double 
     HIPCHK(hipMemcpyAsync(tot, c->d.tot, 8 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(ct, c->d.ct, 10 * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_tally) HIPCHK(hipMemcpyAsync(out_tally, c->d.tally, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_eamp) HIPCHK(hipMemcpyAsync(out_eamp, c->d.eamp, nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_jrad) HIPCHK(hipMemcpyAsync(out_jrad, c->d.jrad, (size_t)nc * ng * 8, hipMemcpyDeviceToHost, c->stream));
     int rc = tm.stop();
     if (rc) return rc;
@@ -628,10 +635,8 @@ int slgpu_transport(slgpu_ctx *c, uint32_t iteration, double *out_tally, double 
     hipEventElapsedTime(&kms, c->ev[12], c->ev[13]);
     c->t_ms[6] += kms;
     tot[2] = -tot[0];     // tot_sflux = -sum e(flux) (particle_advance.f90:283)
-    if (out_totals) {
-        const int map[6] = {0, 1, 2, 3, 4, 5};
-        for (int k = 0; k < 6; ++k) out_totals[k] = tot[map[k]];
-    }
+    if (out_totals)
+        for (int k = 0; k < 7; ++k) out_totals[k] = tot[k];
     if (out_counters)
         for (int k = 0; k < 10; ++k) out_counters[k] = (int64_t)ct[k];
     c->have_transport = true;

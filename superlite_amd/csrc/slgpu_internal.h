@@ -38,6 +38,7 @@ struct Dev {
     int nx, ng, nep, nepg, nion, nel;
     double tauddmc, taulump;
     int puretran, precision;
+    int ampfact, thm_aniso;     // !in_trn_noamp, in_thm_aniso
     int64_t max_steps;
     uint32_t seed;
     // mesh (nx+1 / nx)
@@ -88,6 +89,7 @@ struct Dev {
     double *gcell;                        // 4*nc: dL/dT (means), dL/dsig, dL/dhckt, dL/dhlparr
     double *gfr, *goc;                    // 30*nc, nion*nc element- / ion-major
     double *tally, *jrad, *tot;           // nc, jrad_copies*nc*ng, 8
+    double *eamp;                         // nc grd_eamp
     int jrad_copies;                      // replicas of grd_jrad that spread atomic traffic
     unsigned long long *ct;               // 10
     unsigned long long *queue;            // work counter of the persistent transport

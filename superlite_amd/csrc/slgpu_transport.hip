@@ -144,9 +144,28 @@ template <bool M> __device__ __forceinline__ double p_shell_root(double r1, doub
     }
 }
 
+// transport11.f90:259-285: the experimental anisotropic Thomson cosine for draw r1.  The
+// second cube root has a negative base for r1 below ~0.957, where the real power is NaN;
+// every range test then fails and the isotropic cosine is taken, as in the reference.
+__device__ __forceinline__ double thomson_aniso_mu(double r1)
+{
+    const double q = sqrt(1.0 + 16.0 * ((r1 - 0.5) * (r1 - 0.5)));
+    const double mutemp = pow(4.0 * r1 - 2.0 + q, 1.0 / 3.0) - pow(q - (4.0 - 2.0 * r1), 1.0 / 3.0);
+    const double muhelp = mutemp;
+    const double B = 2 * mutemp * muhelp;
+    const double C = mutemp * mutemp + muhelp * muhelp - 1;
+    const double munew1 = (-B + sqrt(B * B - 4 * C)) / 2;
+    const double munew2 = (-B - sqrt(B * B - 4 * C)) / 2;
+    const bool ok1 = munew1 <= 1 && munew1 >= -1, ok2 = munew2 <= 1 && munew2 >= -1;
+    if (ok1 && ok2) return r1 < 0.5 ? munew1 : munew2;
+    if (ok1) return munew1;
+    if (ok2) return munew2;
+    return 1.0 - 2.0 * r1;
+}
+
 template <bool M>
 __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double &jrd,
-                           double &evelo)
+                           double &eamp, double &evelo)
 {
     const int ng = d.ng, nx = d.nx;
     int ix = p.ix, ig = p.ig;
@@ -154,7 +173,7 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     int ic = ix;
     double r1, r2, db, dcol, dthm, ddop, dd;
     int ixnext;
-    erd = 0.0; jrd = 0.0;
+    erd = 0.0; jrd = 0.0; eamp = 0.0;
     double elabfact = 1.0 - mu * vx * CINV;
     const double elabfactold = elabfact;
     const double xl = XA(ix), xr = XA(ix + 1);
@@ -232,7 +251,8 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     }
     if (dd == dthm || dd == dcol) {
         r1 = s.draw();
-        mu = 1.0 - 2.0 * r1;
+        if (dd == dthm && d.thm_aniso) mu = thomson_aniso_mu(r1);
+        else mu = 1.0 - 2.0 * r1;
         mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
     } else if (dd == db) {
         if (mu >= 0.0 && ix == nx) { p.stat = 1; goto done; }
@@ -305,6 +325,16 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             ix = ix - 1;
         } else {
             mu = (mu - vx * CINV) / (1.0 - vx * mu * CINV);
+            if (d.ampfact && mu < 0.0) {
+                // amplification factor (transport11.f90:410-421)
+                double h = 1.0 / fabs(mu);
+                h = h < 100.0 ? h : 100.0;
+                evelo = evelo - e * 2.0 * (0.55 * h - 1.25 * fabs(mu)) * vx * CINV;
+                eamp = e * 2.0 * 0.55 * (h - 2.0 > 0.0 ? h - 2.0 : 0.0) * vx * CINV;
+                h = h < 2.0 ? h : 2.0;
+                e0 = e0 * (1.0 + 2.0 * (0.55 * h - 1.25 * fabs(mu)) * vx * CINV);
+                e = e * (1.0 + 2.0 * (0.55 * h - 1.25 * fabs(mu)) * vx * CINV);
+            }
             double help = (CAPT(ig, l) + SIGT(l)) * DXX(ix - 1);
             help = 4.0 / (3.0 * help + 6.0 * PC_DEXT);
             r1 = s.draw();
@@ -584,15 +614,15 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
 }
 
 struct Accum {
-    double evelo, eout, edead, estart, esrc;
+    double evelo, eout, edead, estart, esrc, eamp;
     uint32_t nact, nimc, nddmc, nswap, nflux, ndead, nerr, nwarn, nmax;
 };
 
 __device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
 {
-    const double vals[5] = {a.eout, a.evelo, a.estart, a.edead, a.esrc};
-    const int slot[5] = {0, 1, 3, 4, 5};
-    for (int q = 0; q < 5; ++q) {
+    const double vals[6] = {a.eout, a.evelo, a.estart, a.edead, a.esrc, a.eamp};
+    const int slot[6] = {0, 1, 3, 4, 5, 6};
+    for (int q = 0; q < 6; ++q) {
         const double v = wave_sum(vals[q]);
         if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&d.tot[slot[q]], v);
     }
@@ -687,7 +717,12 @@ void k_transport(Dev d, uint32_t iteration)
         int ierr;
         if (imc) {
             a.nimc++;
-            ierr = transport11<M>(d, p, s, erd, jrd, a.evelo);
+            double ea;
+            ierr = transport11<M>(d, p, s, erd, jrd, ea, a.evelo);
+            if (ea != 0.0) {                                   // particle_advance.f90:216-217
+                atomicAdd(&d.eamp[icold - 1], ea);
+                a.eamp += ea;
+            }
             if (p.itype != 1) a.nswap++;
         } else {
             a.nddmc++;

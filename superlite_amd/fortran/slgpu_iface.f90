@@ -19,6 +19,7 @@ module slgpu_iface
     integer(c_int) :: puretran, errorfatal, precision
     integer(c_int32_t) :: seed
     integer(c_int64_t) :: npartmax, max_steps
+    integer(c_int) :: ampfact, thm_aniso
   end type
 
   type, bind(C) :: slgpu_gas
@@ -110,12 +111,12 @@ module slgpu_iface
       integer(c_int64_t), intent(out) :: nnew
     end function
 
-    integer(c_int) function slgpu_transport(ctx, iteration, tally, jrad, totals, counters) &
+    integer(c_int) function slgpu_transport(ctx, iteration, tally, jrad, eamp, totals, counters) &
          bind(C, name="slgpu_transport")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: iteration
-      type(c_ptr), value :: tally, jrad, totals, counters
+      type(c_ptr), value :: tally, jrad, eamp, totals, counters
     end function
 
     integer(c_int) function slgpu_fluxtally(ctx, luminos, lumdev, lumnum) &

@@ -40,7 +40,8 @@ class SlgpuError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("nx", _i), ("ng", _i), ("nep", _i), ("nepg", _i), ("tauddmc", _d),
                 ("taulump", _d), ("puretran", _i), ("errorfatal", _i), ("precision", _i),
-                ("seed", _u32), ("npartmax", _i64), ("max_steps", _i64)]
+                ("seed", _u32), ("npartmax", _i64), ("max_steps", _i64), ("ampfact", _i),
+                ("thm_aniso", _i)]
 
 
 class Gas(ctypes.Structure):
@@ -70,7 +71,7 @@ def lib():
         L.slgpu_opacity_build.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
         L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
         L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
-        L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp]
+        L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _vp]
         L.slgpu_fluxtally.argtypes = [_vp, _vp, _vp, _vp]
         L.slgpu_get_particles.argtypes = [_vp, _i64] + [_vp] * 8
         L.slgpu_sample_groups.argtypes = [_vp, _i, _i, _i64, _vp, _vp, _vp]
@@ -95,11 +96,12 @@ class Context:
     """One slgpu context: device-resident mesh, groups, lines, opacities, particles."""
 
     def __init__(self, nx, ng, nep, nepg, tauddmc=5.0, taulump=10.0, puretran=False,
-                 errorfatal=True, precision=SLGPU_FP64, seed=1, npartmax=1 << 20, max_steps=0):
+                 errorfatal=True, precision=SLGPU_FP64, seed=1, npartmax=1 << 20, max_steps=0,
+                 ampfact=False, thm_aniso=False):
         self.L = lib()
         self.nx, self.ng, self.nep, self.nepg = nx, ng, nep, nepg
         cfg = Config(nx, ng, nep, nepg, tauddmc, taulump, int(puretran), int(errorfatal),
-                     precision, seed, npartmax, max_steps)
+                     precision, seed, npartmax, max_steps, int(ampfact), int(thm_aniso))
         h = _vp()
         self._check(self.L.slgpu_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -187,11 +189,13 @@ class Context:
         nc, ng = self.nx, self.ng
         tally = np.zeros(nc) if want else None
         jrad = np.zeros((nc, ng)) if want else None
-        tot = np.zeros(6)
+        eamp = np.zeros(nc) if want else None
+        tot = np.zeros(7)
         ct = np.zeros(10, np.int64)
-        rc = self.L.slgpu_transport(self.h, iteration, _p(tally), _p(jrad), _p(tot), _p(ct))
-        out = dict(tally=tally, jrad=jrad, eout=tot[0], evelo=tot[1], sflux=tot[2], estart=tot[3],
-                   edead=tot[4], esrc=tot[5], nactive=int(ct[0]), nstepimc=int(ct[1]),
+        rc = self.L.slgpu_transport(self.h, iteration, _p(tally), _p(jrad), _p(eamp), _p(tot),
+                                    _p(ct))
+        out = dict(tally=tally, jrad=jrad, eamp=eamp, eout=tot[0], evelo=tot[1], sflux=tot[2],
+                   estart=tot[3], edead=tot[4], esrc=tot[5], eampsum=tot[6], nactive=int(ct[0]), nstepimc=int(ct[1]),
                    nstepddmc=int(ct[2]), nstepmax=int(ct[3]), nmethswap=int(ct[4]),
                    nflux=int(ct[5]), ndead=int(ct[6]), nerr=int(ct[7]), nwarn=int(ct[8]), rc=rc)
         if check and rc != 0:
@@ -253,12 +257,12 @@ def peak_rates():
 
 
 def setup_workload(w, st, wl, lines, tables, precision=SLGPU_FP64, seed=1, npartmax=None,
-                   puretran=False, errorfatal=True, max_steps=0):
+                   puretran=False, errorfatal=True, max_steps=0, ampfact=False, thm_aniso=False):
     """Create a context holding one workload's static inputs (superlite.f90:87-131)."""
     npm = npartmax if npartmax is not None else (1 << w.n2s)
     ctx = Context(w.nx, w.ng, int(st["nep"]), int(st["nepg"]), w.tauddmc, w.taulump,
                   puretran=puretran, errorfatal=errorfatal, precision=precision, seed=seed,
-                  npartmax=npm, max_steps=max_steps)
+                  npartmax=npm, max_steps=max_steps, ampfact=ampfact, thm_aniso=thm_aniso)
     ctx.set_grid(st["xarr"], st["vxarr"], st["divv"], st["vol"], st["rcell"])
     ctx.set_groups(wl)
     ctx.set_ions(st["ion_iz"], st["ion_ii"])

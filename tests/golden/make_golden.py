@@ -65,6 +65,9 @@ def main():
                     help="only the reference's sourcenumbers split (ref_sourcenumbers.npz)")
     ap.add_argument("--bfxs", action="store_true",
                     help="only the reference's Verner cross sections (ref_bfxs.npz)")
+    ap.add_argument("--mc-modes", action="store_true",
+                    help="only the c1 reference Monte Carlo with the amplification factor and "
+                         "anisotropic Thomson scattering on (ref_c1modes_mc.npz)")
     args = ap.parse_args()
     if args.mc_c2:
         mc_case("c2", 1.0, 10, 8, workload="c2")
@@ -74,6 +77,9 @@ def main():
         return
     if args.bfxs:
         bfxs_cases()
+        return
+    if args.mc_modes:
+        mc_case("c1modes", 1.0, args.mc_n2s, args.mc_iter, ampfact=True, thm_aniso=True)
         return
 
     s0 = RefSession()
@@ -257,13 +263,13 @@ def sourcenumbers_cases():
           sum(r["first"] < r["ns"] for r in rows), sum(r["first"] > r["ns"] for r in rows), flush=True)
 
 
-def mc_case(case, scale, n2s, niter, workload="c1"):
+def mc_case(case, scale, n2s, niter, workload="c1", ampfact=False, thm_aniso=False):
     """Reference transport statistics: niter independent iterations of 2^n2s particles."""
     w = synth.WORKLOADS[workload]
     st = dict(np.load(os.path.join(OUT, f"state_{workload}.npz")))
     s = RefSession()
     g = s.setup(w.nx, w.ng, w.wlmin, w.wlmax, st["xarr"], st["vxarr"], n2s=n2s,
-                nomp=os.cpu_count() or 1)
+                nomp=os.cpu_count() or 1, ampfact=ampfact, thm_aniso=thm_aniso)
     wl = synth.group_grid(w)
     s.put_groups(wl)
     s.put_gas(st["mass"], st["temp"], st["natom"], st["natomfr"], st["nelec"] * scale,
@@ -282,7 +288,7 @@ def mc_case(case, scale, n2s, niter, workload="c1"):
                         lines, synth.load_tables(), flags=0 if scale == 1.0 else 4)
         s.put_opacity(op["cap"], op["sig"], op["capgrey"], op["capross"])
     p = s.prep()
-    keys = ("luminos", "lumnum", "tally", "eout", "evelo", "nstepimc", "nstepddmc",
+    keys = ("luminos", "lumnum", "tally", "eamp", "eout", "evelo", "nstepimc", "nstepddmc",
             "nmethswap", "nflux")
     rows = {k: [] for k in keys}
     for it in range(niter):
@@ -297,7 +303,8 @@ def mc_case(case, scale, n2s, niter, workload="c1"):
             extra = dict(cap=op["cap"], sig=op["sig"], capgrey=op["capgrey"],
                          opaclump=p["opaclump"], emitprob=p["emitprob"], emit=p["emit"])
         np.savez_compressed(os.path.join(OUT, f"ref_{case}_mc.npz"), n2s=n2s, scale=scale,
-                            workload=workload, flags=0 if scale == 1.0 else 4, **extra,
+                            workload=workload, flags=0 if scale == 1.0 else 4,
+                            ampfact=int(ampfact), thm_aniso=int(thm_aniso), **extra,
                             **{k: np.array(v) for k, v in rows.items()})
     s.close()
 

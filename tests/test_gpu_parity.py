@@ -484,6 +484,34 @@ def test_transport_c1_per_particle(S, tables, scale):
     assert abs(tr["eout"] - o["eout"]) / o["eout"] < 2e-2
 
 
+def test_transport_modes_c1_per_particle(S, tables):
+    """The amplification factor and anisotropic Thomson scattering switched on: GPU against
+    the oracle particle by particle, the per-cell grd_eamp tally, and the energy identity
+    e_start = e_out + e_velo + e_dead + sum(grd_eamp)."""
+    say("case: transport c1 per particle, ampfact + thm_aniso")
+    ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 12, seed=19, ampfact=True,
+                                     thm_aniso=True)
+    g = gpu_opacity(ctx, st)
+    p = ctx.ddmc_prep(st["temp"], w.lbol)
+    nvol, nnew = ctx.sources(1 << 12, 4)
+    tr = ctx.transport(4)
+    pp = ctx.particles(nnew)
+    ctx.close()
+    m = oracle_model(st, w, wl, g, prep=p, nvol=nvol, ampfact=True, thm_aniso=True)
+    o = m.iteration(19, 4, per_particle=True)
+    op = o["particles"]
+    assert tr["nerr"] == 0 and o["nerr"] == 0
+    same = (pp["stat"] == op["stat"]) & (pp["ig"] == op["ig"]) & \
+        np.isclose(pp["e"], op["e"], rtol=1e-9) & np.isclose(pp["wl"], op["wl"], rtol=1e-9)
+    say("identical trajectories", same.mean(), "eamp gpu", tr["eampsum"], "oracle", o["eampsum"])
+    assert same.mean() > 0.995
+    assert tr["eampsum"] > 0
+    assert tr["estart"] == pytest.approx(tr["eout"] + tr["evelo"] + tr["edead"] + tr["eampsum"],
+                                         rel=1e-12)
+    np.testing.assert_allclose(tr["eamp"].sum(), tr["eampsum"], rtol=1e-12)
+    np.testing.assert_allclose(tr["eamp"], o["eamp"], rtol=2e-2, atol=1e-3 * o["eamp"].max())
+
+
 def coarse(a, nb=20):
     a = np.asarray(a)
     n = a.shape[-1] - a.shape[-1] % nb

@@ -357,3 +357,34 @@ def test_oracle_transport_vs_reference_statistics(c1, golden, case):
     for key in ("eout", "evelo", "nstepimc", "nstepddmc"):
         z = _zstats(g[key][:, None], [[r[key]] for r in rows])
         assert abs(z[0]) < 4.0, (key, z)
+
+
+def test_oracle_transport_modes_vs_reference_statistics(c1, golden):
+    """The amplification factor (in_trn_noamp = .false., transport11.f90:410-421) and the
+    anisotropic Thomson angle (in_thm_aniso, :258-285) on: the oracle's tallies over 16
+    iterations, including grd_eamp, against 24 iterations of the reference with the same
+    switches (2^14 particles each), and the energy
+    identity e_start = e_out + e_velo + e_dead + sum(grd_eamp)."""
+    import os
+    if not os.path.exists(os.path.join(os.path.dirname(__file__), "golden", "ref_c1modes_mc.npz")):
+        pytest.skip("ref_c1modes_mc.npz not generated (tests/golden/make_golden.py --mc-modes)")
+    w, st, wl, lines = c1
+    g = golden("ref_c1modes_mc.npz")
+    assert int(g["ampfact"]) == 1 and int(g["thm_aniso"]) == 1
+    m = _model(st, w, wl, g, opaclump=g["opaclump"], emitprob=g["emitprob"], emit=g["emit"],
+               nvol=po.sourcenumbers(g["emit"], 1 << int(g["n2s"])), ampfact=True, thm_aniso=True)
+    rows = [m.iteration(seed=13, iteration=it + 1) for it in range(16)]
+    for r in rows:
+        assert r["estart"] == pytest.approx(r["eout"] + r["evelo"] + r["edead"] + r["eampsum"],
+                                            rel=1e-12)
+        assert r["eampsum"] > 0 and r["eamp"].sum() == pytest.approx(r["eampsum"], rel=1e-12)
+        assert r["nerr"] == 0
+    assert np.mean(g["eamp"].sum(1)) > 0
+    for key, f in (("luminos", coarse), ("tally", lambda a: coarse(a, 10)),
+                   ("eamp", lambda a: coarse(a, 10))):
+        z = _zstats(f(g[key]), [f(r[key]) for r in rows])
+        assert np.mean(z ** 2) < 2.5, (key, np.round(z, 2))
+        assert np.max(np.abs(z)) < 4.5, (key, np.round(z, 2))
+    for key in ("eout", "evelo"):
+        z = _zstats(g[key][:, None], [[r[key]] for r in rows])
+        assert abs(z[0]) < 4.0, (key, z)
