@@ -115,6 +115,18 @@ int slgpu_opacity_build(slgpu_ctx *ctx, const slgpu_gas *gas, int flags,
                         double *out_cap64, float *out_cap32, double *out_sig,
                         double *out_capgrey, double *out_capross, float *out_emiss);
 
+/* No counterpart in the reference: slgpu_opacity_build for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as slgpu_opacity_build, but the per-call
+ * staging of the ion populations lives in the library's static 4 MiB device scratch array
+ * instead of a fresh allocation, so it calls no hipMalloc/hipFree.  Limit: 2*nion*nc doubles
+ * must fit in 4 MiB (nion*nc <= 262144), else SLGPU_EINVAL.  The array is shared by every
+ * call on one GPU and keeps the last call's contents: calls that overlap on one GPU (several
+ * contexts or threads) race. */
+int slgpu_opacity_build_noalloc(slgpu_ctx *ctx, const slgpu_gas *gas, int flags,
+                                double *out_cap64, float *out_cap32, double *out_sig,
+                                double *out_capgrey, double *out_capross, float *out_emiss);
+
 /* mpimod_ser.f:98-112 (grd_* = gas_*) + SOURCE/sourceenergy.f90:23-50 +
  * GRID/leakage_opacity11.f90:30-171 + GRID/emission_probability.f90:39-62, plus the
  * per-cell sampling tables the transport uses in place of the reference's

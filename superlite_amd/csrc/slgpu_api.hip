@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,9 +15,40 @@
 
 using namespace slgpu;
 
+// Device memory of slgpu_opacity_build_noalloc: the staging of the per-call ion populations
+// (oc and ginv, 2 * nion * nc doubles), so that entry point needs no allocation.  Each GPU
+// holds its own copy until the process ends; every call on a GPU shares it (calls that
+// overlap on one GPU race).  Host code never uses the array's name as an address: its
+// device address is resolved once per device at slgpu_create (hipGetSymbolAddress) and
+// kept in g_scratch_ptr.
+constexpr size_t SLGPU_SCRATCH_BYTES = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[SLGPU_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device, resolved on first use per device
+hipError_t scratch_address(void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 int fail(int code, const std::string &msg)
 {
@@ -135,7 +167,14 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(SLGPU_EHIP, "no HIP device visible");
+    void *scr = nullptr;
+    {
+        hipError_t e = scratch_address(&scr);
+        if (e != hipSuccess)
+            return fail(SLGPU_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
     slgpu_ctx *c = new slgpu_ctx();
+    c->scratch = scr;
     c->cfg = *cfg;
     c->nx = cfg->nx;
     c->ng = cfg->ng;
@@ -452,7 +491,8 @@ int slgpu_set_ff_table(slgpu_ctx *c, const double *gff)
 namespace {
 
 // The opacity build (O2-O8) with the per-call staging of the ion populations oc and ginv
-// (2 * nion * nc doubles) at `stage`.
+// (2 * nion * nc doubles) at `stage`: a fresh allocation for slgpu_opacity_build, the
+// library's static scratch array for slgpu_opacity_build_noalloc.
 int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *out_cap64,
                        float *out_cap32, double *out_sig, double *out_capgrey,
                        double *out_capross, float *out_emiss, double *stage)
@@ -535,6 +575,22 @@ int slgpu_opacity_build(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *o
     hipStreamSynchronize(c->stream);
     dfree(c, stage);
     return rc;
+}
+
+int slgpu_opacity_build_noalloc(slgpu_ctx *c, const slgpu_gas *gas, int flags,
+                                double *out_cap64, float *out_cap32, double *out_sig,
+                                double *out_capgrey, double *out_capross, float *out_emiss)
+{
+    int rc = check_opacity_inputs(c, gas);
+    if (rc) return rc;
+    const size_t need = (size_t)2 * std::max(c->nion, 1) * c->nx * sizeof(double);
+    if (need > SLGPU_SCRATCH_BYTES)
+        return fail(SLGPU_EINVAL, "opacity_build_noalloc: 2*nion*nc doubles exceed the " +
+                                      std::to_string(SLGPU_SCRATCH_BYTES) + "-byte scratch array");
+    double *stage = (double *)c->scratch;
+    if (!stage) return fail(SLGPU_EORDER, "scratch array address not resolved at slgpu_create");
+    return opacity_build_impl(c, gas, flags, out_cap64, out_cap32, out_sig, out_capgrey,
+                              out_capross, out_emiss, stage);
 }
 
 int slgpu_ddmc_prep(slgpu_ctx *c, const double *radtemp, double lbol, double *out_opaclump,

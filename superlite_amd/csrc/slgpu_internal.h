@@ -123,6 +123,7 @@ struct slgpu_ctx {
     std::vector<double> h_emit_cache;
     int opac_flags = 0;         // flags of the last opacity build (its backward uses them)
     std::vector<void *> allocations;
+    void *scratch = nullptr;    // device address of the library's static scratch array on this context's GPU
     double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[16];
 };

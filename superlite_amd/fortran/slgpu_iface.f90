@@ -92,6 +92,15 @@ module slgpu_iface
       type(c_ptr), value :: cap64, cap32, sig, capgrey, capross, emiss
     end function
 
+    integer(c_int) function slgpu_opacity_build_noalloc(ctx, gas, flags, cap64, cap32, sig, &
+         capgrey, capross, emiss) bind(C, name="slgpu_opacity_build_noalloc")
+      import :: c_int, c_ptr, slgpu_gas
+      type(c_ptr), value :: ctx
+      type(slgpu_gas), intent(in) :: gas
+      integer(c_int), value :: flags
+      type(c_ptr), value :: cap64, cap32, sig, capgrey, capross, emiss
+    end function
+
     integer(c_int) function slgpu_ddmc_prep(ctx, radtemp, lbol, opaclump, emitprob, emit, &
          sthermal) bind(C, name="slgpu_ddmc_prep")
       import :: c_int, c_ptr, c_double

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libslgpu.so")
 
 SLGPU_FP64, SLGPU_MIXED = 0, 1
 NOBB, NOBF, NOFF, NOTHOMSON = 1, 2, 4, 8
-E_TRANSPORT = 4
+SLGPU_EINVAL_CODE, E_TRANSPORT = 1, 4
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -28,7 +28,8 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_set_groups", "slgpu_set_lines", "slgpu_set_bf_tables", "slgpu_set_ff_table",
            "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
-           "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates")
+           "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
+           "slgpu_opacity_build_noalloc")
 
 
 class SlgpuError(RuntimeError):
@@ -69,6 +70,7 @@ def lib():
         L.slgpu_set_ff_table.argtypes = [_vp, _vp]
         L.slgpu_set_ions.argtypes = [_vp, _i, _vp, _vp]
         L.slgpu_opacity_build.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
+        L.slgpu_opacity_build_noalloc.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
         L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
         L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
         L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _vp]
@@ -147,9 +149,10 @@ class Context:
 
     # -- per iteration ----------------------------------------------------
     def opacity_build(self, temp, natom, vol, mass, nelec, natomfr, oc, ginv, flags=0,
-                      want=("cap64", "cap", "sig", "capgrey", "capross", "emiss")):
+                      want=("cap64", "cap", "sig", "capgrey", "capross", "emiss"), noalloc=False):
         """physical_opacity + opacity_planckmean + opacity_rossmean.
-        natomfr (30, nc); oc, ginv (nc, nion).  Group-resolved outputs have shape (nc, ng)."""
+        natomfr (30, nc); oc, ginv (nc, nion).  Group-resolved outputs have shape (nc, ng).
+        noalloc: through slgpu_opacity_build_noalloc (static scratch, no device allocation)."""
         nc, ng = self.nx, self.ng
         keep = [_c(temp, np.float64), _c(natom, np.float64), _c(vol, np.float64),
                 _c(mass, np.float64), _c(nelec, np.float64), _c(np.asarray(natomfr).T, np.float64),
@@ -161,10 +164,10 @@ class Context:
         if "emiss" in want: out["emiss"] = np.zeros((nc, ng), np.float32)
         for k in ("sig", "capgrey", "capross"):
             if k in want: out[k] = np.zeros(nc)
-        self._check(self.L.slgpu_opacity_build(self.h, ctypes.byref(gas), flags,
-                                               _p(out.get("cap64")), _p(out.get("cap")),
-                                               _p(out.get("sig")), _p(out.get("capgrey")),
-                                               _p(out.get("capross")), _p(out.get("emiss"))))
+        fn = self.L.slgpu_opacity_build_noalloc if noalloc else self.L.slgpu_opacity_build
+        self._check(fn(self.h, ctypes.byref(gas), flags, _p(out.get("cap64")), _p(out.get("cap")),
+                       _p(out.get("sig")), _p(out.get("capgrey")), _p(out.get("capross")),
+                       _p(out.get("emiss"))))
         return out
 
     def ddmc_prep(self, radtemp, lbol, want=True):
