@@ -175,6 +175,36 @@ int slgpu_opacity_vjp(slgpu_ctx *ctx, const double *grad_cap64, const double *gr
                       const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
                       double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc);
 
+/* LTE EOS data of the host, as its ionsmod.f ion_el holds them after its first EOS call:
+ * ni[30] = ion_el(iz)%ni (stages incl. the bare nucleus); e, q, g1 [30*31], element-major
+ * ([(iz-1)*31 + ii-1]): ionisation energy [erg], partition function (frozen at the first
+ * call, ionsmod.f:208-215) and ground-level weight glev(1) of stage ii; amass[30] =
+ * elem_data(iz)%m [amu] (elemdatamod.f). */
+int slgpu_set_eos_data(slgpu_ctx *ctx, const int *ni, const double *e, const double *q,
+                       const double *g1, const double *amass);
+
+/* GAS/gas_setup.f:101-160 (massfr2natomfr, no container fraction) + GAS/eos_update.f:23-45 +
+ * ionsmod.f:160-356 (ions_solve_eos, LTE) on the device, for every cell with mass > 0.
+ * temp, mass (nc); massfr (30*nc, iz-fastest); nelec (nc, in/out: the start value of the
+ * secant solve, 1.0 at the first call as gasmod.f:101, then the previous solution).  acc:
+ * relative accuracy of nelec (<= 0: the reference's 1e-8).  Outputs (any may be NULL):
+ * natom (nc), natomfr (30*nc), oc/ginv (nion*nc, the layout of slgpu_gas.oc for the ion
+ * list of slgpu_set_ions), niter (nc; 41 = accuracy not reached, the reference warns).
+ * SLGPU_EINVAL where the reference stops (ionsmod.f:291,338-349; gas_setup.f:120-123). */
+int slgpu_eos(slgpu_ctx *ctx, const double *temp, const double *mass, const double *massfr,
+              double acc, double *nelec, double *out_natom, double *out_natomfr,
+              double *out_oc, double *out_ginv, int *out_niter);
+
+/* Differentiation form: vector-Jacobian product of the last slgpu_eos at its solution (the
+ * implicit function theorem on the converged nelec, partition functions held fixed).
+ * dL/d(natom (nc), natomfr (30*nc), nelec (nc), oc (nion*nc)) in, any may be NULL (zero);
+ * out (any may be NULL): the EOS's part of dL/dT (nc), dL/dmass (nc; dL/drho = vol*dL/dmass),
+ * dL/dmassfr (30*nc).  Chained after slgpu_opacity_vjp this gives the opacity's gradients
+ * with respect to temperature, density and mass fractions. */
+int slgpu_eos_vjp(slgpu_ctx *ctx, const double *grad_natom, const double *grad_natomfr,
+                  const double *grad_nelec, const double *grad_oc, double *out_grad_temp,
+                  double *out_grad_mass, double *out_grad_massfr);
+
 /* Verification hook for the transport's group samplers; no single reference routine
  * corresponds.  It evaluates, for n (draw r, cell) pairs, the sampler the transport
  * uses: table 0-3 = the lumped loops of diffusion11.f90 (left leak :375-398, right leak
@@ -187,8 +217,8 @@ int slgpu_sample_groups(slgpu_ctx *ctx, int table, int direct, int64_t n, const 
 
 /* accumulated device time per phase in ms, measured with HIP events on the context
  * stream: [0] opacity, [1] ddmc prep, [2] sources, [3] transport, [4] fluxtally,
- * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone, [7] opacity backward;
- * reset = 1 zeroes them */
+ * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone, [7] opacity backward,
+ * [8] EOS, [9] EOS backward; n <= 10; reset = 1 zeroes them */
 int slgpu_timers(slgpu_ctx *ctx, double *ms, int n, int reset);
 
 /* the HIP stream all work of the context is ordered on (hipStream_t) */

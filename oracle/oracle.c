@@ -6,6 +6,7 @@
 #include "oracle.h"
 #include "philox.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,6 +26,7 @@
 #define PC_EV 1.60217653e-12
 #define PC_ANG 1e-8
 #define PC_MBARN 1e-18
+#define PC_AMU 1.66053886e-24
 
 static const double CINV = 1.0 / PC_C;
 
@@ -490,6 +492,123 @@ int or_emitgroup(const or_model *m, double r, int ic)
         if (emitprob > r1) break;
     }
     return ig;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LTE EOS.  ionsmod.f:311-354 saha_nelec: per element the Saha recursion from stage istart
+ * (restarted one stage higher while the last stage overflows; the stages below keep their
+ * values), normalised; returns sum_z natomfr(z) * (mean charge of z). */
+#define EOS_E(iz, ii) (ed->e[((iz) - 1) * 31 + (ii) - 1])
+#define EOS_Q(iz, ii) (ed->q[((iz) - 1) * 31 + (ii) - 1])
+#define EOS_G(iz, ii) (ed->g1[((iz) - 1) * 31 + (ii) - 1])
+static int or_saha_nelec(const or_eos_data *ed, const double *natomfr, double kti,
+                         double sahac2, double *n /* 30*31 */, double *nelec_new)
+{
+    double ne_sum = 0.0;
+    for (int iz = 1; iz <= 30; ++iz) {
+        const int nion = ed->ni[iz - 1];
+        double *nz = n + (iz - 1) * 31;
+        double help = 0.0;
+        for (int istart = 1; istart <= nion - 1; ++istart) {
+            nz[istart - 1] = 1.0;
+            for (int ii = istart + 1; ii <= nion; ++ii) {
+                help = sahac2 * exp(-kti * EOS_E(iz, ii - 1)) * nz[ii - 2] * EOS_Q(iz, ii) / EOS_Q(iz, ii - 1);
+                nz[ii - 1] = help;
+            }
+            if (help < DBL_MAX) break;
+        }
+        double nsum = 0.0;
+        for (int ii = 1; ii <= nion; ++ii) nsum += nz[ii - 1];
+        if (nsum != nsum || nsum > DBL_MAX || nsum < DBL_MIN) return 1;   /* :338-349 stop */
+        nsum = 1.0 / nsum;
+        for (int ii = 1; ii <= nion; ++ii) nz[ii - 1] = nz[ii - 1] * nsum;
+        double ne = 0.0;
+        for (int ii = 2; ii <= nion; ++ii) ne = ne + (ii - 1) * nz[ii - 1];
+        ne_sum = ne_sum + natomfr[iz - 1] * ne;
+    }
+    *nelec_new = ne_sum;
+    return 0;
+}
+
+/* ionsmod.f:160-309 ions_solve_eos: secant iteration on nelec with the reference's bracket
+ * bookkeeping; n receives the populations of the last Saha evaluation */
+static int or_solve_eos(const or_eos_data *ed, const double *natomfr, double temp, double ndens,
+                        double acc, double *nelec, double *n, int *niter)
+{
+    const double kti = 1.0 / (PC_KB * temp);
+    const double sahac = pow(PC_PI2 * PC_ME * PC_KB * temp / (PC_H * PC_H), 1.5);
+    double nel[2] = {0.0, 0.0}, err2[2] = {1e10, -1e10};
+    int iprev = 0, iconv;
+    for (iconv = 1; iconv <= 40; ++iconv) {
+        const double sahac2 = 2.0 / (*nelec * ndens) * sahac;
+        double nelec_new;
+        if (or_saha_nelec(ed, natomfr, kti, sahac2, n, &nelec_new)) return 1;
+        const double err = *nelec / nelec_new - 1.0;
+        if (fabs(err) < acc) {
+            if (iconv > 1) *nelec = nelec_new;
+            break;
+        }
+        int ih = (err * err2[1] > err * err2[0]) ? 1 : 0;     /* maxloc(err*nec%err) */
+        nel[ih] = *nelec;
+        err2[ih] = err;
+        if (iconv == 1) {
+            double x = 1.5 * nelec_new - 0.5 * *nelec;
+            x = fmin(3.0 * nelec_new, x);
+            x = fmax(0.33 * nelec_new, x);
+            *nelec = x;
+        } else {
+            ih = (fabs(err2[1]) > fabs(err2[0])) ? 2 : 1;      /* maxloc(abs(nec%err)) */
+            const double help = fabs(err2[0] / err2[1]);
+            double ynew;
+            if (ih != iprev && help < 1e5 && help > 1e-5) ynew = 0.0;
+            else ynew = 0.001 * err2[ih - 1];
+            iprev = ih;
+            const double dxdy = (nel[1] - nel[0]) / (err2[1] - err2[0]);
+            *nelec = nel[0] - dxdy * (err2[0] - ynew);
+            if (*nelec <= 0.0) return 1;
+        }
+    }
+    *niter = iconv;
+    return 0;
+}
+
+int or_eos(int nc, const or_eos_data *ed, const double *temp, const double *mass,
+           const double *vol, const double *massfr, double acc, double *nelec, double *natom,
+           double *natomfr, int nion, const int *ion_iz, const int *ion_ii, double *oc,
+           double *ginv, int *niter)
+{
+    if (!(acc > 0.0)) acc = 1e-8;                  /* ionsmod.f:183 */
+    int rc = 0;
+    for (int i = 0; i < nc; ++i) {
+        double *fr = natomfr + (size_t)i * 30;
+        niter[i] = 0;
+        if (mass[i] <= 0.0) continue;
+        /* gas_setup.f:125-157 massfr2natomfr */
+        const double *mf = massfr + (size_t)i * 30;
+        double help = 0.0;
+        for (int l = 0; l < 30; ++l) help = help + mf[l];
+        double na = 0.0;
+        for (int l = 0; l < 30; ++l) {
+            fr[l] = (mf[l] / help) * mass[i];
+            fr[l] = fr[l] / (ed->amass[l] * PC_AMU);
+        }
+        for (int l = 0; l < 30; ++l) na = na + fr[l];
+        natom[i] = na;
+        for (int l = 0; l < 30; ++l) fr[l] = fr[l] / na;
+        /* eos_update.f:25-32 */
+        const double ndens = natom[i] / vol[i];
+        double n[30 * 31];
+        memset(n, 0, sizeof n);
+        if (or_solve_eos(ed, fr, temp[i], ndens, acc, &nelec[i], n, &niter[i])) { rc = 1; continue; }
+        /* eos_update.f:36-45, for the listed ions */
+        for (int k = 0; k < nion; ++k) {
+            const int iz = ion_iz[k], ii = ion_ii[k];
+            ginv[(size_t)i * nion + k] = 1.0 / EOS_G(iz, ii);
+            oc[(size_t)i * nion + k] = natom[i] * fr[iz - 1] * EOS_G(iz, ii) * n[(iz - 1) * 31 + ii - 1] /
+                                       (EOS_Q(iz, ii) * vol[i]);
+        }
+    }
+    return rc;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -96,6 +96,27 @@ typedef struct {
 int or_opacity(const or_opac_in *in, double *cap64 /* ng*nc group-fastest */, float *cap32,
                double *sig, double *capgrey, double *capross, float *emiss);
 
+/* ---- LTE EOS: gas_setup.f:101-160 (massfr2natomfr, no container fraction) +
+ *      eos_update.f:23-45 + ionsmod.f:160-356 (ions_solve_eos, saha_nelec) ---- */
+typedef struct {
+    const int *ni;          /* 30: ionisation stages of element iz (ion_el(iz)%ni, incl. the bare nucleus) */
+    const double *e;        /* 30*31, [iz-1][ii-1]: ionisation energy of stage ii [erg] */
+    const double *q;        /* partition function, frozen at the host's first EOS call (H6) */
+    const double *g1;       /* ground-level statistical weight glev(1) */
+    const double *amass;    /* 30 element masses [amu] (elemdatamod.f) */
+} or_eos_data;
+
+/* Per massive cell: natom and natomfr from mass and mass fractions, then the Saha/secant
+ * solve for nelec (in/out: the start value, H5) to relative accuracy acc (the reference's
+ * 1e-8 when acc <= 0), then the ground-level populations oc/ginv of the listed ions.
+ * massfr, natomfr: 30*nc iz-fastest; oc, ginv: nion*nc ion-fastest; niter: iterations used
+ * (41 = not converged).  Returns 0, or 1 when the reference would stop (nelec <= 0 or an
+ * invalid Saha normalisation). */
+int or_eos(int nc, const or_eos_data *ed, const double *temp, const double *mass,
+           const double *vol, const double *massfr, double acc, double *nelec, double *natom,
+           double *natomfr, int nion, const int *ion_iz, const int *ion_ii, double *oc,
+           double *ginv, int *niter);
+
 /* ---- DDMC prep: leakage_opacity11.f90:30-171, emission_probability.f90:39-62 ---- */
 void or_leakage(const or_model *m, double *opaclump);
 void or_emission_probability(const or_model *m, double *emitprob);

@@ -47,6 +47,8 @@ def lib():
         _lib.or_opacity.argtypes = [_vp] + [_vp] * 6
         _lib.or_leakage.argtypes = [_vp, _vp]
         _lib.or_emission_probability.argtypes = [_vp, _vp]
+        _lib.or_eos.restype = _i
+        _lib.or_eos.argtypes = [_i, _vp] + [_vp] * 4 + [_d] + [_vp] * 3 + [_i] + [_vp] * 5
     return _lib
 
 
@@ -143,6 +145,31 @@ def opacity(wl, temp, natom, vol, mass, nelec, natomfr, ion_iz, ion_ii, oc, ginv
     lib().or_opacity(ctypes.byref(s), _p(cap64), _p(cap32), _p(sig), _p(capgrey),
                      _p(capross), _p(emiss))
     return dict(cap64=cap64, cap=cap32, sig=sig, capgrey=capgrey, capross=capross, emiss=emiss)
+
+
+class _EosData(ctypes.Structure):
+    _fields_ = [("ni", _vp), ("e", _vp), ("q", _vp), ("g1", _vp), ("amass", _vp)]
+
+
+def eos(ed, temp, mass, vol, massfr, ion_iz, ion_ii, nelec=None, acc=0.0):
+    """massfr2natomfr + eos_update + ions_solve_eos (LTE) on every massive cell.
+    ed: dict ni (30), e/q/g1 (30, 31), amass (30); massfr (30, nc).  nelec: start values
+    (1.0, gasmod.f:101, if None).  Returns natom, natomfr (30, nc), nelec, oc/ginv (nc, nion),
+    niter and the return code."""
+    nc = len(temp)
+    keep = dict(ni=_c(ed["ni"], np.int32), e=_c(ed["e"], np.float64), q=_c(ed["q"], np.float64),
+                g1=_c(ed["g1"], np.float64), amass=_c(ed["amass"], np.float64))
+    d = _EosData(*[_p(keep[k]) for k in ("ni", "e", "q", "g1", "amass")])
+    ne = np.ones(nc) if nelec is None else np.array(nelec, np.float64, copy=True)
+    nion = len(ion_iz)
+    natom, natomfr = np.zeros(nc), np.zeros((nc, 30))
+    oc, ginv, niter = np.zeros((nc, nion)), np.zeros((nc, nion)), np.zeros(nc, np.int32)
+    args = [_c(temp, np.float64), _c(mass, np.float64), _c(vol, np.float64),
+            _c(np.asarray(massfr).T, np.float64)]
+    iz, ii = _c(ion_iz, np.int32), _c(ion_ii, np.int32)
+    rc = lib().or_eos(nc, ctypes.byref(d), *[_p(a) for a in args], float(acc), _p(ne), _p(natom),
+                      _p(natomfr), nion, _p(iz), _p(ii), _p(oc), _p(ginv), _p(niter))
+    return dict(natom=natom, natomfr=natomfr.T.copy(), nelec=ne, oc=oc, ginv=ginv, niter=niter, rc=rc)
 
 
 class Model:

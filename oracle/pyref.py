@@ -121,6 +121,60 @@ class RefSession:
         self.lib.slref_get_gas(_i(nc), _p(natom), _p(natomfr), _p(nelec), _p(vol))
         return natom, natomfr.T.copy(), nelec, vol
 
+    def read_par(self, deckdir):
+        """the reference's namelist reader on deckdir/input.par: the parameters the hot path uses"""
+        ndim, fl = np.zeros(3, np.int32), np.zeros(3, np.int32)
+        iv, rv, lv = np.zeros(6, np.int32), np.zeros(9), np.zeros(10, np.int32)
+        cwd = os.getcwd()
+        try:
+            os.chdir(deckdir)
+            self.lib.slref_read_par(_p(ndim), _p(fl), _p(iv), _p(rv), _p(lv))
+        finally:
+            os.chdir(cwd)
+        return dict(ndim=ndim, flx_ndim=fl, ivals=iv, rvals=rv, lvals=lv)
+
+    def read_str(self, deckdir, nx):
+        """the reference's input.str reader (1D spherical) on deckdir/input.str"""
+        nab = _i()
+        flags = np.zeros(4, np.int32)
+        cwd = os.getcwd()
+        try:
+            os.chdir(deckdir)
+            self.lib.slref_read_str(_i(nx), ctypes.byref(nab), _p(flags))
+        finally:
+            os.chdir(cwd)
+        na = nab.value
+        out = dict(xleft=np.zeros(nx + 1), vxleft=np.zeros(nx + 1), mass=np.zeros(nx),
+                   temp=np.zeros(nx), massfr=np.zeros((nx, na)), iabund=np.zeros(na, np.int32),
+                   vol=np.zeros(nx), radtemp=np.zeros(nx), lum=np.zeros(nx), ye=np.zeros(nx))
+        self.lib.slref_get_str(_i(nx), _i(na), *[_p(out[k]) for k in
+                                                ("xleft", "vxleft", "mass", "temp", "massfr", "iabund",
+                                                 "vol", "radtemp", "lum", "ye")])
+        out["flags"] = flags
+        return out
+
+    def write_outputs(self, outdir, a, opacdump=True):
+        """the reference's output_flux + output_grid writers, one iteration, into outdir"""
+        nc, ng = self.nc, self.ng
+        keys = ("luminos", "lumdev", "lumnum", "nvol", "tempinv", "radtemp", "capgrey", "capross",
+                "sig", "tally", "vol", "tot")
+        arrs = [np.ascontiguousarray(a[k], np.int32 if k in ("lumnum", "nvol") else np.float64)
+                for k in keys]
+        cwd = os.getcwd()
+        try:
+            os.chdir(outdir)
+            self.lib.slref_write_outputs(_i(nc), _i(ng), *[_p(x) for x in arrs], _i(int(opacdump)))
+        finally:
+            os.chdir(cwd)
+
+    def eos_data(self):
+        """ni(30), e/q/g1 (30, 31) per element and stage, element masses (30) [amu]"""
+        ni = np.zeros(30, np.int32)
+        e, q, g1 = np.zeros((30, 31)), np.zeros((30, 31)), np.zeros((30, 31))
+        am = np.zeros(30)
+        self.lib.slref_get_eos_data(_p(ni), _p(e), _p(q), _p(g1), _p(am))
+        return dict(ni=ni, e=e, q=q, g1=g1, amass=am)
+
     def get_pops(self, ion_iz, ion_ii):
         iz = np.ascontiguousarray(ion_iz, np.int32)
         ii = np.ascontiguousarray(ion_ii, np.int32)

@@ -171,6 +171,32 @@ contains
     vol = gas_vol
   end subroutine
 
+  !-- the LTE EOS data the host holds after its first eos_update: per element the number of
+  !   ionisation stages ni (incl. the bare nucleus), per stage the ionisation energy e [erg],
+  !   the partition function q frozen at the first call (ionsmod.f:208-215) and the ground
+  !   level's statistical weight glev(1); and the element masses [amu] (elemdatamod.f)
+  subroutine slref_get_eos_data(ni, e, q, g1, am) bind(C, name="slref_get_eos_data")
+    use ionsmod, only: ion_el
+    use elemdatamod, only: elem_data
+    integer(c_int), intent(out) :: ni(30)
+    real(c_double), intent(out) :: e(31, 30), q(31, 30), g1(31, 30), am(30)
+    integer :: iz, ii
+    e = 0d0
+    q = 0d0
+    g1 = 0d0
+    do iz = 1, 30
+      ni(iz) = ion_el(iz)%ni
+      am(iz) = elem_data(iz)%m
+      do ii = 1, ion_el(iz)%ni
+        e(ii, iz) = ion_el(iz)%i(ii)%e
+        q(ii, iz) = ion_el(iz)%i(ii)%q
+        if (allocated(ion_el(iz)%i(ii)%glev)) then
+          if (size(ion_el(iz)%i(ii)%glev) > 0) g1(ii, iz) = ion_el(iz)%i(ii)%glev(1)
+        endif
+      enddo
+    enddo
+  end subroutine
+
   !-- ground-level populations of the listed ions, (nion, nc) ion-fastest
   subroutine slref_get_pops(nc, nion, ion_iz, ion_ii, oc, ginv) bind(C, name="slref_get_pops")
     use ionsmod, only: ion_grndlev
@@ -397,6 +423,91 @@ contains
     out_tot = [tot_eout, tot_evelo, tot_sflux, tot_sthermal]
     out_ct = [int(src_nnew, 8), ct_npactive(1), ct_npstepimc(1), ct_npstepddmc(1), &
          ct_npstepmax(1), ct_npmethswap(1), ct_npflux(1), 0_8]
+  end subroutine
+
+  !-- the reference's namelist reader (inputparmod.f:216-231) on ./input.par, and the values
+  !   of the parameters the hot path consumes
+  subroutine slref_read_par(ndim, flxndim, ivals, rvals, lvals) bind(C, name="slref_read_par")
+    use inputparmod
+    integer(c_int), intent(out) :: ndim(3), flxndim(3), ivals(6)
+    real(c_double), intent(out) :: rvals(9)
+    integer(c_int), intent(out) :: lvals(10)
+    call read_inputpars
+    ndim = in_ndim
+    flxndim = in_flx_ndim
+    ivals = [in_grp_ng, in_src_n2s, in_src_ns, in_niter, in_nomp, in_grd_igeom]
+    rvals = [in_grp_wlmin, in_grp_wlmax, in_flx_wlmin, in_flx_wlmax, in_L_bol, &
+         in_trn_tauddmc, in_taulump, in_srcepwr, in_L_inner]
+    lvals = merge(1, 0, [in_puretran, in_trn_errorfatal, in_trn_noamp, in_thm_aniso, &
+         in_nobbopac, in_nobfopac, in_noffopac, in_nothmson, in_nlte, in_noeos])
+  end subroutine
+
+  !-- the reference's structure-file reader (inputstrmod.f:85-274) on ./input.str, 1D spherical
+  subroutine slref_read_str(nx, nabund, flags) bind(C, name="slref_read_str")
+    use inputstrmod
+    integer(c_int), value :: nx
+    integer(c_int), intent(out) :: nabund, flags(4)
+    call read_inputstr(11, [nx, 1, 1], 1)
+    nabund = str_nabund
+    flags = merge(1, 0, [str_lvol, str_lradtemp, str_llum, str_lye])
+  end subroutine
+
+  subroutine slref_get_str(nx, nabund, xleft, vxleft, mass, temp, massfr, iabund, vol, radtemp, &
+       lum, ye) bind(C, name="slref_get_str")
+    use inputstrmod
+    integer(c_int), value :: nx, nabund
+    real(c_double), intent(out) :: xleft(nx+1), vxleft(nx+1), mass(nx), temp(nx), &
+         massfr(nabund, nx), vol(nx), radtemp(nx), lum(nx), ye(nx)
+    integer(c_int), intent(out) :: iabund(nabund)
+    xleft = str_xleft
+    vxleft = str_vxleft
+    mass = str_mass(:, 1, 1)
+    temp = str_temp(:, 1, 1)
+    massfr = str_massfr(:, :, 1, 1)
+    iabund = str_iabund
+    vol = 0d0
+    radtemp = 0d0
+    lum = 0d0
+    ye = 0d0
+    if (str_lvol) vol = str_vol(:, 1, 1)
+    if (str_lradtemp) radtemp = str_radtemp(:, 1, 1)
+    if (str_llum) lum = str_lum(:, 1, 1)
+    if (str_lye) ye = str_ye(:, 1, 1)
+  end subroutine
+
+  !-- the reference's writers on given values, into the current directory:
+  !   output_flux (OUTPUT/output_flux.f:3-66) and output_grid (OUTPUT/output_grid.f:3-170),
+  !   one iteration per call (the first call writes the headers and the grid files)
+  subroutine slref_write_outputs(nc, ng, luminos, lumdev, lumnum, nvol, tempinv, radtemp, &
+       capgrey, capross, sig, tally, vol, tot, opacdump) bind(C, name="slref_write_outputs")
+    use fluxmod, only: flx_luminos, flx_lumdev, flx_lumnum
+    use gridmod
+    use totalsmod
+    use inputparmod, only: in_io_opacdump, in_io_nogriddump
+    integer(c_int), value :: nc, ng, opacdump
+    real(c_double), intent(in) :: luminos(ng), lumdev(ng), tempinv(nc), radtemp(nc), capgrey(nc), &
+         capross(nc), sig(nc), tally(nc), vol(nc), tot(4)
+    integer(c_int), intent(in) :: lumnum(ng), nvol(nc)
+    flx_luminos(:, 1, 1) = luminos
+    flx_lumdev(:, 1, 1) = lumdev
+    flx_lumnum(:, 1, 1) = lumnum
+    grd_nvol = nvol
+    grd_tempinv = tempinv
+    grd_radtemp = radtemp
+    grd_capgrey = capgrey
+    grd_capemitgrey = 0d0
+    grd_capross = capross
+    grd_sig = sig
+    grd_tally = tally
+    grd_vol = vol
+    tot_eout = tot(1)
+    tot_evelo = tot(2)
+    tot_sflux = tot(3)
+    tot_sthermal = tot(4)
+    in_io_nogriddump = .false.
+    in_io_opacdump = merge('one ', 'off ', opacdump /= 0)
+    call output_flux
+    call output_grid
   end subroutine
 
 end module slref_harness

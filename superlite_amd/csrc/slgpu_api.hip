@@ -196,7 +196,7 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
         delete c;
         return fail(SLGPU_EHIP, "hipStreamCreate failed");
     }
-    for (int k = 0; k < 16; ++k) hipEventCreate(&c->ev[k]);
+    for (int k = 0; k < 20; ++k) hipEventCreate(&c->ev[k]);
     const size_t nc = c->nx, ng = c->ng, cg = nc * ng, np = cfg->npartmax;
     int rc = 0;
     rc |= dalloc(c, (double **)&d.xarr, nc + 1);
@@ -273,6 +273,19 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.dev, ng);
     rc |= dalloc(c, &d.num, ng);
     rc |= dalloc(c, &d.flags, 1);
+    rc |= dalloc(c, &d.eos_ni, 30);
+    rc |= dalloc(c, &d.eos_e, 30 * 31);
+    rc |= dalloc(c, &d.eos_q, 30 * 31);
+    rc |= dalloc(c, &d.eos_g1, 30 * 31);
+    rc |= dalloc(c, &d.eos_amass, 30);
+    rc |= dalloc(c, &d.eos_sahac2, nc);
+    rc |= dalloc(c, &c->eos_temp, nc);
+    rc |= dalloc(c, &c->eos_mass, nc);
+    rc |= dalloc(c, &c->eos_massfr, 30 * nc);
+    rc |= dalloc(c, &c->eos_nelec, nc);
+    rc |= dalloc(c, &c->eos_natom, nc);
+    rc |= dalloc(c, &c->eos_natomfr, 30 * nc);
+    rc |= dalloc(c, &c->eos_niter, nc);
     if (rc) {
         std::string m = g_err;
         slgpu_destroy(c);
@@ -289,7 +302,7 @@ int slgpu_destroy(slgpu_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : c->allocations) hipFree(p);
     c->allocations.clear();
-    for (int k = 0; k < 16; ++k) hipEventDestroy(c->ev[k]);
+    for (int k = 0; k < 20; ++k) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -364,6 +377,10 @@ int slgpu_set_ions(slgpu_ctx *c, int nion, const int *ion_iz, const int *ion_ii)
     c->d.nion = nion;
     c->h_ion_iz.assign(ion_iz, ion_iz + nion);
     c->h_ion_ii.assign(ion_ii, ion_ii + nion);
+    dfree(c, c->d.ion_z);
+    dfree(c, c->d.ion_i);
+    dfree(c, c->eos_oc);
+    dfree(c, c->eos_ginv);
     dfree(c, c->d.ocg);
     dfree(c, c->d.ocT);
     dfree(c, c->d.ginvT);
@@ -376,8 +393,18 @@ int slgpu_set_ions(slgpu_ctx *c, int nion, const int *ion_iz, const int *ion_ii)
     rc |= dalloc(c, &c->d.goc, (size_t)std::max(nion, 1) * nc);
     rc |= dalloc(c, &c->d.ocT, (size_t)(nion + 30) * nc);     // tail: ff coordinates per element
     rc |= dalloc(c, (double **)&c->d.bfxs, (size_t)std::max(nion, 1) * c->ng);
+    rc |= dalloc(c, &c->d.ion_z, std::max(nion, 1));
+    rc |= dalloc(c, &c->d.ion_i, std::max(nion, 1));
+    rc |= dalloc(c, &c->eos_oc, (size_t)std::max(nion, 1) * nc);
+    rc |= dalloc(c, &c->eos_ginv, (size_t)std::max(nion, 1) * nc);
     if (rc) return rc;
+    if (nion > 0) {
+        HIPCHK(hipMemcpyAsync(c->d.ion_z, ion_iz, nion * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d.ion_i, ion_ii, nion * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     c->have_ions = true;
+    c->have_eos = false;
     if (c->have_lines && c->have_groups) rc = upload_lines(c);
     if (!rc && c->have_bf && c->have_groups) rc = build_bfxs(c);
     return rc;
@@ -801,10 +828,109 @@ int slgpu_sample_groups(slgpu_ctx *c, int table, int direct, int64_t n, const do
     return rc ? fail(SLGPU_EHIP, "sample_groups: device error") : 0;
 }
 
+int slgpu_set_eos_data(slgpu_ctx *c, const int *ni, const double *e, const double *q,
+                       const double *g1, const double *amass)
+{
+    if (!c || !ni || !e || !q || !g1 || !amass) return fail(SLGPU_EINVAL, "null argument");
+    for (int iz = 1; iz <= 30; ++iz) {
+        const int n = ni[iz - 1];
+        if (n < 1 || n > iz + 1) return fail(SLGPU_EINVAL, "ni(iz) must be in 1..iz+1 (ionsmod.f:102)");
+        if (!(amass[iz - 1] > 0.0)) return fail(SLGPU_EINVAL, "element masses must be positive");
+        for (int ii = 1; ii <= n; ++ii)
+            if (!(q[(iz - 1) * 31 + ii - 1] > 0.0))
+                return fail(SLGPU_EINVAL, "partition functions must be positive for every stage");
+    }
+    HIPCHK(hipMemcpyAsync(c->d.eos_ni, ni, 30 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.eos_e, e, 30 * 31 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.eos_q, q, 30 * 31 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.eos_g1, g1, 30 * 31 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.eos_amass, amass, 30 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_eos_data = true;
+    c->have_eos = false;
+    return 0;
+}
+
+int slgpu_eos(slgpu_ctx *c, const double *temp, const double *mass, const double *massfr,
+              double acc, double *nelec, double *out_natom, double *out_natomfr, double *out_oc,
+              double *out_ginv, int *out_niter)
+{
+    if (!c || !temp || !mass || !massfr || !nelec) return fail(SLGPU_EINVAL, "null argument");
+    if (!(c->have_grid && c->have_ions && c->have_eos_data))
+        return fail(SLGPU_EORDER, "grid, ions and EOS data must be set first");
+    const int nc = c->nx, nion = c->nion;
+    for (int i = 0; i < nc; ++i) {
+        if (mass[i] <= 0.0) continue;
+        if (!(temp[i] > 0.0) || !(nelec[i] > 0.0))
+            return fail(SLGPU_EINVAL, "temperatures and start electron fractions must be positive");
+        double s = 0.0;
+        for (int l = 0; l < 30; ++l) {
+            if (massfr[(size_t)i * 30 + l] < 0.0) return fail(SLGPU_EINVAL, "massfr2natomfr: negative mass fractions");
+            s += massfr[(size_t)i * 30 + l];
+        }
+        if (!(s > 0.0)) return fail(SLGPU_EINVAL, "massfr2natomfr: all mass fractions zero");
+    }
+    if (!(acc > 0.0)) acc = 1e-8;                            // ionsmod.f:183
+    PhaseTimer tm(c, 8);
+    HIPCHK(hipMemsetAsync(c->d.flags, 0, 4, c->stream));
+    HIPCHK(hipMemcpyAsync(c->eos_temp, temp, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->eos_mass, mass, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->eos_massfr, massfr, (size_t)30 * nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->eos_nelec, nelec, nc * 8, hipMemcpyHostToDevice, c->stream));
+    launch_eos(c, c->eos_temp, c->eos_mass, c->eos_massfr, acc, c->eos_nelec, c->eos_natom,
+               c->eos_natomfr, c->eos_oc, c->eos_ginv, c->eos_niter);
+    HIPCHK(hipGetLastError());
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, c->d.flags, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(nelec, c->eos_nelec, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_natom) HIPCHK(hipMemcpyAsync(out_natom, c->eos_natom, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_natomfr) HIPCHK(hipMemcpyAsync(out_natomfr, c->eos_natomfr, (size_t)30 * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_oc && nion > 0) HIPCHK(hipMemcpyAsync(out_oc, c->eos_oc, (size_t)nion * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_ginv && nion > 0) HIPCHK(hipMemcpyAsync(out_ginv, c->eos_ginv, (size_t)nion * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_niter) HIPCHK(hipMemcpyAsync(out_niter, c->eos_niter, nc * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int rc = tm.stop();
+    if (rc) return rc;
+    if (flag) {
+        c->have_eos = false;
+        return fail(SLGPU_EINVAL, "ions_solve_eos: nelec <= 0 or invalid Saha normalisation (ionsmod.f:291,338-349)");
+    }
+    c->have_eos = true;
+    return 0;
+}
+
+int slgpu_eos_vjp(slgpu_ctx *c, const double *grad_natom, const double *grad_natomfr,
+                  const double *grad_nelec, const double *grad_oc, double *out_grad_temp,
+                  double *out_grad_mass, double *out_grad_massfr)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_eos) return fail(SLGPU_EORDER, "slgpu_eos must run first");
+    const int nc = c->nx, nion = c->nion;
+    PhaseTimer tm(c, 9);
+    double *buf = nullptr;
+    const size_t nin = (size_t)nc * (2 + 30 + std::max(nion, 1)), nout = (size_t)nc * 32;
+    int rc = dalloc(c, &buf, nin + nout);
+    if (rc) return rc;
+    double *gna = buf, *gne = buf + nc, *gfr = buf + 2 * (size_t)nc, *goc = buf + 32 * (size_t)nc;
+    double *oT = buf + nin, *om = oT + nc, *omf = om + nc;
+    if (grad_natom) HIPCHK(hipMemcpyAsync(gna, grad_natom, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (grad_nelec) HIPCHK(hipMemcpyAsync(gne, grad_nelec, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (grad_natomfr) HIPCHK(hipMemcpyAsync(gfr, grad_natomfr, (size_t)30 * nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (grad_oc && nion > 0) HIPCHK(hipMemcpyAsync(goc, grad_oc, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
+    launch_eos_vjp(c, c->eos_temp, c->eos_mass, c->eos_massfr, c->eos_nelec, c->eos_natom,
+                   c->eos_natomfr, c->eos_oc, gna, gfr, gne, goc, oT, om, omf);
+    HIPCHK(hipGetLastError());
+    if (out_grad_temp) HIPCHK(hipMemcpyAsync(out_grad_temp, oT, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_grad_mass) HIPCHK(hipMemcpyAsync(out_grad_mass, om, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (out_grad_massfr) HIPCHK(hipMemcpyAsync(out_grad_massfr, omf, (size_t)30 * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    rc = tm.stop();
+    dfree(c, buf);
+    return rc;
+}
+
 int slgpu_timers(slgpu_ctx *c, double *ms, int n, int reset)
 {
     if (!c) return fail(SLGPU_EINVAL, "null context");
-    for (int k = 0; k < n && k < 8; ++k) ms[k] = c->t_ms[k];
+    for (int k = 0; k < n && k < 10; ++k) ms[k] = c->t_ms[k];
     if (reset)
         for (double &t : c->t_ms) t = 0.0;
     return 0;

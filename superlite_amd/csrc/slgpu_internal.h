@@ -96,6 +96,12 @@ struct Dev {
     double *lum, *dev;                    // ng
     int *num;                             // ng
     int *flags;                           // bad input flags raised by kernels
+    // LTE EOS: the host's ion data (ionsmod.f ion_el) and the state of the last solve
+    int *ion_z, *ion_i;                   // nion: the ion list of slgpu_set_ions on the device
+    int *eos_ni;                          // 30
+    double *eos_e, *eos_q, *eos_g1;       // 30*31 [iz-1][ii-1]
+    double *eos_amass;                    // 30
+    double *eos_sahac2;                   // nc: Saha constant of the last evaluation (0: not solved)
 };
 
 struct Timer {
@@ -123,9 +129,14 @@ struct slgpu_ctx {
     std::vector<double> h_emit_cache;
     int opac_flags = 0;         // flags of the last opacity build (its backward uses them)
     std::vector<void *> allocations;
+    bool have_eos_data = false, have_eos = false;
+    // inputs and outputs of the last slgpu_eos, kept on the device for slgpu_eos_vjp
+    double *eos_temp = nullptr, *eos_mass = nullptr, *eos_massfr = nullptr, *eos_nelec = nullptr,
+           *eos_natom = nullptr, *eos_natomfr = nullptr, *eos_oc = nullptr, *eos_ginv = nullptr;
+    int *eos_niter = nullptr;
     void *scratch = nullptr;    // device address of the library's static scratch array on this context's GPU
-    double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev[16];
+    double t_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[20];
 };
 
 namespace slgpu {
@@ -150,4 +161,12 @@ void launch_fluxtally(slgpu_ctx *c);
 void launch_jrad_reduce(slgpu_ctx *c);
 void launch_sample_groups(slgpu_ctx *c, int t, int direct, int64_t n, const double *r,
                           const int *cell, int *out);
+void launch_eos(slgpu_ctx *c, const double *temp, const double *mass, const double *massfr,
+                double acc, double *nelec, double *natom, double *natomfr, double *oc,
+                double *ginv, int *niter);
+void launch_eos_vjp(slgpu_ctx *c, const double *temp, const double *mass, const double *massfr,
+                    const double *nelec, const double *natom, const double *natomfr,
+                    const double *oc, const double *g_natom, const double *g_fr,
+                    const double *g_ne, const double *g_oc, double *out_T, double *out_mass,
+                    double *out_mf);
 }  // namespace slgpu

@@ -125,11 +125,22 @@ template <bool M> __device__ __forceinline__ double p_path(double r, double k)
     if constexpr (M) return -(double)__logf((float)r) / k;
     else return -log(r) / k;
 }
-// exp(-divv*t/3): the DDMC adiabatic factor (diffusion11.f90:623-629)
+// exp(-divv*t/3): the DDMC adiabatic factor (diffusion11.f90:623-629).  A particle takes
+// ~1e4-1e5 of these factors, so a systematic last-bit error in an FP32 factor near 1
+// compounds into a percent-level energy bias.  The precision form therefore evaluates the
+// loss fraction d = 1 - exp(-a) in FP32 to FP32 *relative* accuracy (series for small a)
+// and forms the factor 1 - d in FP64.
 template <bool M> __device__ __forceinline__ double p_adiabatic(double divv, double t)
 {
-    if constexpr (M) return (double)__expf((float)(-divv * t) * (1.0f / 3.0f));
-    else return exp(-divv * t / 3);
+    if constexpr (M) {
+        const float a = (float)(divv * t) * (1.0f / 3.0f);
+        const float d = a < 0.03f
+                            ? a * (1.0f - a * (0.5f - a * (1.0f / 6.0f - a * (1.0f / 24.0f))))
+                            : 1.0f - expf(-a);
+        return 1.0 - (double)d;
+    } else {
+        return exp(-divv * t / 3);
+    }
 }
 // r^3-uniform radius in [xl, xr]: (r*xr^3 + (1-r)*xl^3)**(1.0/3.0) with the reference's
 // single-precision exponent (diffusion11.f90:336,597).  Radii cubed (~1e47 cm^3) overflow

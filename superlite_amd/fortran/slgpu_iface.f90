@@ -165,6 +165,32 @@ module slgpu_iface
       integer(c_int), intent(out) :: group(*)
     end function
 
+    integer(c_int) function slgpu_set_eos_data(ctx, ni, e, q, g1, amass) &
+         bind(C, name="slgpu_set_eos_data")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: ni(*)
+      real(c_double), intent(in) :: e(*), q(*), g1(*), amass(*)
+    end function
+
+    integer(c_int) function slgpu_eos(ctx, temp, mass, massfr, acc, nelec, natom, natomfr, &
+         oc, ginv, niter) bind(C, name="slgpu_eos")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: temp(*), mass(*), massfr(*)
+      real(c_double), value :: acc
+      real(c_double), intent(inout) :: nelec(*)
+      type(c_ptr), value :: natom, natomfr, oc, ginv, niter
+    end function
+
+    integer(c_int) function slgpu_eos_vjp(ctx, grad_natom, grad_natomfr, grad_nelec, grad_oc, &
+         grad_temp, grad_mass, grad_massfr) bind(C, name="slgpu_eos_vjp")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: grad_natom, grad_natomfr, grad_nelec, grad_oc
+      type(c_ptr), value :: grad_temp, grad_mass, grad_massfr
+    end function
+
     integer(c_int) function slgpu_timers(ctx, ms, n, reset) bind(C, name="slgpu_timers")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -29,7 +29,7 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
            "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
-           "slgpu_opacity_build_noalloc")
+           "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp")
 
 
 class SlgpuError(RuntimeError):
@@ -82,6 +82,9 @@ def lib():
         L.slgpu_stream.restype = _vp
         L.slgpu_stream.argtypes = [_vp]
         L.slgpu_peak_rates.argtypes = [_vp, _vp]
+        L.slgpu_set_eos_data.argtypes = [_vp] * 6
+        L.slgpu_eos.argtypes = [_vp, _vp, _vp, _vp, _d] + [_vp] * 6
+        L.slgpu_eos_vjp.argtypes = [_vp] * 8
         _lib = L
     return _lib
 
@@ -108,6 +111,7 @@ class Context:
         self._check(self.L.slgpu_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
         self.npart = 0
+        self.builds = 0          # counts opacity builds and EOS solves (state the backward uses)
 
     def _check(self, rc):
         if rc != 0:
@@ -164,6 +168,7 @@ class Context:
         if "emiss" in want: out["emiss"] = np.zeros((nc, ng), np.float32)
         for k in ("sig", "capgrey", "capross"):
             if k in want: out[k] = np.zeros(nc)
+        self.builds += 1
         fn = self.L.slgpu_opacity_build_noalloc if noalloc else self.L.slgpu_opacity_build
         self._check(fn(self.h, ctypes.byref(gas), flags, _p(out.get("cap64")), _p(out.get("cap")),
                        _p(out.get("sig")), _p(out.get("capgrey")), _p(out.get("capross")),
@@ -233,6 +238,43 @@ class Context:
         out["oc"] = out["oc"][:, : self.nion]
         return out
 
+    def set_eos_data(self, ed):
+        """the host's LTE EOS data: ni (30), e/q/g1 (30, 31), amass (30)"""
+        a = [_c(ed["ni"], np.int32), _c(ed["e"], np.float64), _c(ed["q"], np.float64),
+             _c(ed["g1"], np.float64), _c(ed["amass"], np.float64)]
+        self._check(self.L.slgpu_set_eos_data(self.h, *[_p(x) for x in a]))
+
+    def eos(self, temp, mass, massfr, nelec=None, acc=0.0):
+        """massfr2natomfr + eos_update (LTE) on the device.  massfr (30, nc); nelec: start
+        values (1.0 if None).  Returns natom, natomfr (30, nc), nelec, oc/ginv (nc, nion), niter."""
+        nc = self.nx
+        ne = np.ones(nc) if nelec is None else np.array(nelec, np.float64, copy=True)
+        out = dict(natom=np.zeros(nc), natomfr=np.zeros((nc, 30)),
+                   oc=np.zeros((nc, max(self.nion, 1))), ginv=np.zeros((nc, max(self.nion, 1))),
+                   niter=np.zeros(nc, np.int32))
+        self.builds += 1
+        keep = [_c(temp, np.float64), _c(mass, np.float64), _c(np.asarray(massfr).T, np.float64)]
+        self._check(self.L.slgpu_eos(self.h, *[_p(k) for k in keep], float(acc), _p(ne),
+                                     *[_p(out[k]) for k in ("natom", "natomfr", "oc", "ginv", "niter")]))
+        out["nelec"] = ne
+        out["natomfr"] = out["natomfr"].T.copy()
+        out["oc"], out["ginv"] = out["oc"][:, : self.nion], out["ginv"][:, : self.nion]
+        return out
+
+    def eos_vjp(self, grad_natom=None, grad_natomfr=None, grad_nelec=None, grad_oc=None):
+        """dL/d(temp, mass, massfr (30, nc)) of the last eos() for the given dL/d(natom,
+        natomfr (30, nc), nelec, oc (nc, nion)); temp is the EOS's part only"""
+        nc = self.nx
+        g = [None if grad_natom is None else _c(grad_natom, np.float64),
+             None if grad_natomfr is None else _c(np.asarray(grad_natomfr).T, np.float64),
+             None if grad_nelec is None else _c(grad_nelec, np.float64),
+             None if grad_oc is None else _c(grad_oc, np.float64)]
+        out = dict(temp=np.zeros(nc), mass=np.zeros(nc), massfr=np.zeros((nc, 30)))
+        self._check(self.L.slgpu_eos_vjp(self.h, *[_p(x) for x in g],
+                                         *[_p(out[k]) for k in ("temp", "mass", "massfr")]))
+        out["massfr"] = out["massfr"].T.copy()
+        return out
+
     def sample_groups(self, table, r, cell, direct=False):
         """group sampled by table (0-3 lumped L/R/D/S, 4 emitgroup) for draws r in cells"""
         r = np.ascontiguousarray(r, np.float64)
@@ -243,10 +285,11 @@ class Context:
         return out
 
     def timers(self, reset=False):
-        ms = np.zeros(8)
-        self._check(self.L.slgpu_timers(self.h, _p(ms), 8, int(reset)))
+        ms = np.zeros(10)
+        self._check(self.L.slgpu_timers(self.h, _p(ms), 10, int(reset)))
         return dict(opacity=ms[0], prep=ms[1], sources=ms[2], transport=ms[3], fluxtally=ms[4],
-                    opacity_kernel=ms[5], transport_kernel=ms[6], opacity_vjp=ms[7])
+                    opacity_kernel=ms[5], transport_kernel=ms[6], opacity_vjp=ms[7], eos=ms[8],
+                    eos_vjp=ms[9])
 
 
 def peak_rates():

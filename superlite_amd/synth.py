@@ -8,8 +8,12 @@ bbxsmod.f:27-34 (wl0 [A] f32, gxs f32, chilw [1/cm] f32, iz i16, ii i16).
 
 The ionisation balance (gas_natom, gas_natomfr, gas_nelec, ion_grndlev oc/ginv) is an
 input of the opacity build, produced by the host's LTE EOS (GAS/eos_update.f:23-45).
-For the named workloads it is loaded from the committed per-workload files under
-tests/golden/ (written by tests/golden/make_golden.py with SuperLite's own EOS).
+For the named workloads it is loaded from the per-workload files in superlite_amd/data/,
+which hold what a SuperLite host holds after its set-up: the atomic tables as its readers
+parse them (tables.npz), the ion data of its EOS (eos_<w>.npz: ionisation energies,
+partition functions frozen at the first EOS call, ground-level weights, element masses) and
+its EOS state for the workload (state_<w>.npz).  tests/golden/make_golden.py writes them
+from SuperLite's own code.
 """
 from __future__ import annotations
 
@@ -32,8 +36,7 @@ T_SNAP = 20.0 * PC_DAY          # 20 days, homologous snapshot time
 V_BREAK = 5000e5                # 5000 km/s, density break
 M_TOT = 1.4 * PC_MSUN
 
-GOLDEN_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                          "tests", "golden")
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 @dataclasses.dataclass
@@ -157,13 +160,19 @@ def line_list(w: Workload, ion_iz: np.ndarray, ion_ii: np.ndarray):
 def load_tables():
     """Verner bf fit tables (6,7,30,30)/(7,30,30) f32 and Sutherland g_ff (81,41) f64,
     Fortran order, as parsed by SuperLite's own readers (bfxsmod.f:25-61, ffxsmod.f:28-54)."""
-    d = np.load(os.path.join(GOLDEN_DIR, "tables.npz"))
+    d = np.load(os.path.join(DATA_DIR, "tables.npz"))
     return d["bf_ph1"], d["bf_ph2"], d["ff_gff"]
 
 
 def load_state(name: str):
     """Host gas state of a named workload (ionisation balance from SuperLite's EOS)."""
-    return dict(np.load(os.path.join(GOLDEN_DIR, f"state_{name}.npz")))
+    return dict(np.load(os.path.join(DATA_DIR, f"state_{name}.npz")))
+
+
+def load_eos_data(name: str):
+    """The host's LTE EOS data for a named workload (ionsmod.f ion_el after the first
+    ions_solve_eos call, elemdatamod.f masses): ni (30), e/q/g1 (30, 31), amass (30)."""
+    return dict(np.load(os.path.join(DATA_DIR, f"eos_{name}.npz")))
 
 
 def build(name: str):

@@ -112,18 +112,28 @@ def test_precision_form_c2(S, tables):
             f"mean relative bias {np.mean(bias):+.2e} +- {np.std(bias) / np.sqrt(len(bias)):.1e}")
         assert np.mean(z ** 2) < chi2_bound(len(z))
         assert np.abs(z).max() < zmax_bound(len(z))
-    # per group where each precision's iterations escaped enough packets
+    # per group where both precisions escaped at least 100 packets over the iterations
     num0 = np.sum([r["num"] for r in r0], 0)
     num1 = np.sum([r["num"] for r in r1], 0)
-    g = (num0 > 400) & (num1 > 400)
+    g = (num0 >= 100) & (num1 >= 100)
+    assert g.sum() >= 100
     z, _, _ = zstats([r["lum"][g] for r in r1], [r["lum"][g] for r in r0])
     say(f"per group ({g.sum()} groups): chi2/dof {np.mean(z ** 2):.2f}, max|z| {np.abs(z).max():.2f}")
     assert np.mean(z ** 2) < chi2_bound(len(z))
     assert np.abs(z).max() < zmax_bound(len(z))
-    for key in ("eout", "evelo"):
-        z, ma, mb = zstats([[r[key]] for r in r1], [[r[key]] for r in r0])
-        say(f"{key}: mixed/fp64 - 1 = {ma[0] / mb[0] - 1:+.2e}, z {z[0]:+.2f}")
-        assert abs(z[0]) < 4.0
+    # bias separated from noise: each iteration runs both precisions on the same seeds, so
+    # the per-iteration ratio mixed/fp64 - 1 cancels the common Monte Carlo noise; its mean
+    # over the iterations is the bias, its standard error the residual noise
+    for key, f in (("eout", lambda r: r["eout"]), ("evelo", lambda r: r["evelo"]),
+                   ("total shell tally", lambda r: r["tally"].sum()),
+                   ("inner half tally", lambda r: r["tally"][: w.nx // 2].sum()),
+                   ("outer half tally", lambda r: r["tally"][w.nx // 2:].sum()),
+                   ("blue half spectrum", lambda r: r["lum"][: w.ng // 2].sum()),
+                   ("red half spectrum", lambda r: r["lum"][w.ng // 2:].sum())):
+        d = np.array([f(a) / f(b) - 1.0 for a, b in zip(r1, r0)])
+        m, se = d.mean(), d.std(ddof=1) / np.sqrt(len(d))
+        say(f"{key}: paired bias mixed/fp64 - 1 = {m:+.2e} +- {se:.1e}")
+        assert abs(m) < 1e-2, key
 
 
 # ---------------------------------------------------------------------------
@@ -283,7 +293,8 @@ def test_transport_newton_doppler_per_particle(S, tables):
     assert tr["nerr"] == 0 and o["nerr"] == 0
     assert tr["nstepimc"] + tr["nstepddmc"] > 0
     same = _compare(pp, o["particles"], "NR branch")
-    assert same.mean() > 0.995
+    assert same.all()
+    assert (tr["nstepimc"], tr["nstepddmc"]) == (o["nstepimc"], o["nstepddmc"])
 
 
 def test_transport_c2_per_particle(S, tables):
@@ -298,7 +309,7 @@ def test_transport_c2_per_particle(S, tables):
         "capped", o["nerr"], "gpu capped", tr["nerr"])
     assert set(np.unique(pp["err"])) <= {0, 200}
     same = _compare(pp, o["particles"], "c2")
-    assert same.mean() > 0.99
+    assert same.all()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
@@ -317,4 +328,4 @@ def test_transport_c5(S, tables, prec):
     if prec == "fp64":
         o = _oracle_model(st, w, wl, g, prep, nvol, max_steps=3000).iteration(29, 1, per_particle=True)
         same = _compare(pp, o["particles"], "c5")
-        assert same.mean() > 0.99
+        assert same.all()

@@ -1,0 +1,165 @@
+"""The device LTE EOS (slgpu_eos) and its adjoint (slgpu_eos_vjp), through the C ABI:
+
+  - the forward against the oracle's restatement (itself equal to the reference's own
+    eos_update on these inputs) on the three workloads, first and second calls;
+  - the gradient of the group opacity with respect to temperature, density and element mass
+    fractions (device EOS -> opacity build, backward through slgpu_opacity_vjp and
+    slgpu_eos_vjp) against central differences at h = 1e-4 relative: c1, and the 400-shell
+    workload on 64 random (shell, parameter) probes over T, rho and its 8 mass fractions;
+  - the precision form's gradients against FP64's, and the torch.autograd.Function.
+
+The differences solve the EOS to 1e-14 (the reference's 1e-8 would put 1e-8 noise into
+every perturbed solution); the adjoint holds at the exact solution.  Marked gpu."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from superlite_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def S():
+    from superlite_amd import slgpu
+    return slgpu
+
+
+def _ctx(S, name, tables, **kw):
+    w, st, wl, lines = synth.build(name)
+    ctx = S.setup_workload(w, st, wl, lines, tables, npartmax=1024, **kw)
+    ctx.set_eos_data(synth.load_eos_data(name))
+    return ctx, w, st, wl, lines
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c5"])
+def test_eos_forward_vs_oracle(S, tables, name):
+    say(f"case: device EOS {name}")
+    ctx, w, st, wl, lines = _ctx(S, name, tables)
+    g = ctx.eos(st["temp"], st["mass"], st["massfr"])
+    o = po.eos(synth.load_eos_data(name), st["temp"], st["mass"], st["vol"], st["massfr"],
+               st["ion_iz"], st["ion_ii"])
+    same_it = np.mean(g["niter"] == o["niter"])
+    say(f"{name}: iterations identical in {same_it:.4f} of the cells, max {g['niter'].max()}")
+    assert same_it == 1.0
+    for k in ("natom", "natomfr", "nelec", "oc", "ginv"):
+        rel = np.max(np.abs(g[k] - o[k]) / np.maximum(np.abs(o[k]), 1e-300))
+        say(f"  {k}: max rel vs oracle {rel:.2e}")
+        assert rel < 1e-12, k
+    # the reference's own first call (the committed workload state)
+    for k in ("natom", "natomfr", "nelec", "oc", "ginv"):
+        np.testing.assert_allclose(g[k], st[k], rtol=1e-12, atol=0, err_msg=k)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "c5"])
+def test_eos_second_call_vs_reference(S, tables, golden, name):
+    say(f"case: device EOS second call {name}")
+    gg = golden("ref_eos.npz")
+    ctx, w, st, wl, lines = _ctx(S, name, tables)
+    g = ctx.eos(gg[f"{name}_temp"], gg[f"{name}_mass"], gg[f"{name}_massfr"], nelec=gg[f"{name}_nelec0"])
+    for k in ("natom", "natomfr", "nelec", "oc", "ginv"):
+        np.testing.assert_allclose(g[k], gg[f"{name}_{k}"], rtol=1e-12, atol=0, err_msg=k)
+    ctx.close()
+
+
+def _forward(ctx, st, temp, rho, massfr, nelec0, flags=0):
+    mass = rho * st["vol"]
+    e = ctx.eos(temp, mass, massfr, nelec=nelec0, acc=1e-14)
+    return ctx.opacity_build(temp, e["natom"], st["vol"], mass, e["nelec"], e["natomfr"], e["oc"],
+                             e["ginv"], flags=flags, want=("cap64",))["cap64"], e
+
+
+def _grad(ctx, st, temp, rho, massfr, nelec0, Wc):
+    _, e = _forward(ctx, st, temp, rho, massfr, nelec0)
+    g = ctx.opacity_vjp(grad_cap64=Wc)
+    ge = ctx.eos_vjp(g["natom"], g["natomfr"], g["nelec"], g["oc"])
+    return dict(temp=g["temp"] + ge["temp"], rho=ge["mass"] * st["vol"], massfr=ge["massfr"]), e
+
+
+def _fd_probes(ctx, st, probes, Wc, base, h=1e-4):
+    worst = 0.0
+    nel0 = base[3]["nelec"]
+    for kind, idx in probes:
+        vals = []
+        for sgn in (1.0, -1.0):
+            a = {"temp": base[0].copy(), "rho": base[1].copy(), "massfr": base[2].copy()}
+            a[kind][idx] *= 1.0 + sgn * h
+            vals.append(_forward(ctx, st, a["temp"], a["rho"], a["massfr"], nel0)[0])
+        dx = {"temp": base[0], "rho": base[1], "massfr": base[2]}[kind][idx] * h
+        fd = float(np.sum(Wc * (vals[0] - vals[1]))) / (2 * dx)
+        an = base[4][kind][idx]
+        err = abs(fd - an) / max(abs(fd), abs(an), 1e-300)
+        worst = max(worst, err)
+        say(f"  {kind:7s} {str(idx):10s} fd {fd: .6e} adjoint {an: .6e} rel {err:.1e}")
+    return worst
+
+
+def _setup_grad(ctx, st, rng):
+    temp, rho, mf = st["temp"].copy(), st["mass"] / st["vol"], st["massfr"].copy()
+    cap, e = _forward(ctx, st, temp, rho, mf, None)
+    Wc = rng.standard_normal(cap.shape) / np.abs(cap[cap > 0]).mean()
+    g, e = _grad(ctx, st, temp, rho, mf, e["nelec"], Wc)
+    return (temp, rho, mf, e, g), Wc
+
+
+def test_gas_opacity_gradient_c1(S, tables):
+    """c1: every cell's T and rho, and the mass fractions of its 4 elements on 10 cells
+    (H has zero mass fraction at c1: its derivative exists and is checked too)"""
+    say("case: opacity gradient w.r.t. T, rho, X through the device EOS, c1")
+    ctx, w, st, wl, lines = _ctx(S, "c1", tables)
+    rng = np.random.default_rng(41)
+    base, Wc = _setup_grad(ctx, st, rng)
+    cells = rng.choice(w.nx, 10, replace=False)
+    probes = [("temp", (int(i),)) for i in cells] + [("rho", (int(i),)) for i in cells]
+    probes += [("massfr", (z - 1, int(i))) for i in cells[:5] for z in (2, 8, 26)]
+    worst = _fd_probes(ctx, st, probes, Wc, base)
+    say("worst rel error", worst)
+    assert worst < 1e-4
+    # autograd.Function: the same gradients
+    import torch
+    from superlite_amd import autograd as sag
+    temp, rho, mf, e, g = base
+    ts = [torch.tensor(a, requires_grad=True) for a in (temp, rho, mf)]
+    cap, grey, ross = sag.gas_opacity(ctx, dict(vol=st["vol"], nelec0=e["nelec"], acc=1e-14), *ts)
+    (cap * torch.from_numpy(Wc)).sum().backward()
+    for t, k in zip(ts, ("temp", "rho", "massfr")):
+        np.testing.assert_allclose(t.grad.numpy(), g[k], rtol=1e-10, atol=1e-300, err_msg=k)
+    ctx.close()
+
+
+def test_gas_opacity_gradient_c2_64_probes(S, tables):
+    """The differentiation form at the 400-shell size (10^4 groups, 2e6 lines): 64 random
+    (shell, parameter) probes over T, rho and the 8 mass fractions, central differences at
+    h = 1e-4 relative against the adjoint to 1e-4; then the precision form's gradients
+    against FP64's (within 1e-5 of the largest of each kind)."""
+    say("case: opacity gradient w.r.t. T, rho, X through the device EOS, c2, 64 probes")
+    ctx, w, st, wl, lines = _ctx(S, "c2", tables)
+    rng = np.random.default_rng(1003)
+    base, Wc = _setup_grad(ctx, st, rng)
+    elems = list(w.elements)
+    probes = []
+    for _ in range(64):
+        i = int(rng.integers(w.nx))
+        p = int(rng.integers(10))
+        probes.append(("temp", (i,)) if p == 0 else (("rho", (i,)) if p == 1 else ("massfr", (elems[p - 2] - 1, i))))
+    worst = _fd_probes(ctx, st, probes, Wc, base)
+    say("worst rel error", worst)
+    assert worst < 1e-4
+    g64 = base[4]
+    ctx.close()
+    ctx, w, st, wl, lines = _ctx(S, "c2", tables, precision=S.SLGPU_MIXED)
+    temp, rho, mf, e, _ = base
+    gm, _ = _grad(ctx, st, temp, rho, mf, e["nelec"], Wc)
+    for k in ("temp", "rho", "massfr"):
+        err = np.abs(gm[k] - g64[k]).max() / np.abs(g64[k]).max()
+        say(f"precision form {k}: max|mixed - fp64| / max|fp64| = {err:.2e}")
+        assert err < 1e-5, k
+    ctx.close()

@@ -466,7 +466,10 @@ def test_transport_c1_per_particle(S, tables, scale):
         np.isclose(pp["e"], op["e"], rtol=1e-9) & np.isclose(pp["wl"], op["wl"], rtol=1e-9)
     say("identical trajectories", same.mean(), "steps gpu", tr["nstepimc"], tr["nstepddmc"],
         "oracle", o["nstepimc"], o["nstepddmc"])
-    assert same.mean() > 0.995
+    # same discrete outcomes for every particle (exp/log differ by ulps between the device
+    # and the host, which has not flipped a comparison on any tested particle)
+    assert same.all() and np.array_equal(pp["nstep"], op["nstep"])
+    assert (tr["nstepimc"], tr["nstepddmc"]) == (o["nstepimc"], o["nstepddmc"])
     # energy bookkeeping: every particle's start energy ends as escaped, adiabatic or dead
     assert tr["estart"] == pytest.approx(tr["eout"] + tr["evelo"] + tr["edead"], rel=1e-12)
     assert tr["esrc"] == pytest.approx(o["esrc"], rel=1e-12)
@@ -504,7 +507,7 @@ def test_transport_modes_c1_per_particle(S, tables):
     same = (pp["stat"] == op["stat"]) & (pp["ig"] == op["ig"]) & \
         np.isclose(pp["e"], op["e"], rtol=1e-9) & np.isclose(pp["wl"], op["wl"], rtol=1e-9)
     say("identical trajectories", same.mean(), "eamp gpu", tr["eampsum"], "oracle", o["eampsum"])
-    assert same.mean() > 0.995
+    assert same.all()
     assert tr["eampsum"] > 0
     assert tr["estart"] == pytest.approx(tr["eout"] + tr["evelo"] + tr["edead"] + tr["eampsum"],
                                          rel=1e-12)

@@ -388,3 +388,57 @@ def test_oracle_transport_modes_vs_reference_statistics(c1, golden):
     for key in ("eout", "evelo"):
         z = _zstats(g[key][:, None], [[r[key]] for r in rows])
         assert abs(z[0]) < 4.0, (key, z)
+
+
+# --- LTE EOS (ionsmod.f:160-356, eos_update.f:23-45, gas_setup.f:101-160) ----------
+@pytest.mark.parametrize("name", ["c1", "c2", "c5"])
+def test_eos_vs_reference_first_call(name):
+    """or_eos from nelec = 1 (gasmod.f:101) reproduces the reference's own first EOS call on
+    each workload: natom, natomfr, nelec and the ground-level populations oc/ginv."""
+    st = synth.load_state(name)
+    ed = synth.load_eos_data(name)
+    r = po.eos(ed, st["temp"], st["mass"], st["vol"], st["massfr"], st["ion_iz"], st["ion_ii"])
+    assert r["rc"] == 0 and np.all(r["niter"] <= 40)
+    for k in ("natom", "natomfr", "nelec", "oc", "ginv"):
+        np.testing.assert_allclose(r[k], st[k], rtol=1e-13, atol=0, err_msg=k)
+
+
+@pytest.mark.parametrize("name", ["c1", "c5"])
+def test_eos_vs_reference_second_call(golden, name):
+    """A second EOS call with changed temperatures, masses and mass fractions: nelec starts
+    from the first call's solution (H5) and the partition functions stay those of the first
+    call (H6), as the reference's own second call does."""
+    g = golden("ref_eos.npz")
+    st = synth.load_state(name)
+    ed = synth.load_eos_data(name)
+    r = po.eos(ed, g[f"{name}_temp"], g[f"{name}_mass"], st["vol"], g[f"{name}_massfr"],
+               st["ion_iz"], st["ion_ii"], nelec=g[f"{name}_nelec0"])
+    assert r["rc"] == 0
+    for k in ("natom", "natomfr", "nelec", "oc", "ginv"):
+        np.testing.assert_allclose(r[k], g[f"{name}_{k}"], rtol=1e-13, atol=0, err_msg=k)
+
+
+def test_eos_saha_closed_form():
+    """Pure hydrogen, stages H I / H II with q = g1 = 1: the Saha solution has the closed form
+    x^2/(1-x) = 2 sahac(T) exp(-chi/kT) / ndens, ne = x; the secant solve at a tight
+    accuracy reaches it."""
+    ed = dict(ni=np.zeros(30, np.int32), e=np.zeros((30, 31)), q=np.zeros((30, 31)),
+              g1=np.zeros((30, 31)), amass=np.ones(30))
+    ed["ni"][:] = 2
+    ed["e"][:, 0] = 13.6 * 1.60217653e-12
+    ed["q"][:, :2] = 1.0
+    ed["g1"][:, :2] = 1.0
+    kb, h, me, amu = 1.3806505e-16, 6.6260693e-27, 9.1093826e-28, 1.66053886e-24
+    temp = np.array([6000.0, 8000.0, 10000.0, 14000.0])
+    vol = np.ones(4)
+    mass = np.array([1e-10, 1e-12, 1e-13, 1e-11])
+    mf = np.zeros((30, 4))
+    mf[0] = 1.0
+    r = po.eos(ed, temp, mass, vol, mf, np.array([1], np.int32), np.array([1], np.int32), acc=1e-14)
+    ndens = r["natom"] / vol
+    sahac = (2 * np.pi * me * kb * temp / h ** 2) ** 1.5
+    c = 2 * sahac * np.exp(-ed["e"][0, 0] / (kb * temp)) / ndens
+    x = 2 * c / (c + np.sqrt(c * c + 4 * c))                     # x^2 = c (1 - x), stable root
+    np.testing.assert_allclose(r["nelec"], x, rtol=1e-12)
+    # ground-state population of H I: natom (1 - x) / V, with 1 - x = x^2 / c
+    np.testing.assert_allclose(r["oc"][:, 0], r["natom"] * (x * x / c) / vol, rtol=1e-10)
