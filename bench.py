@@ -159,39 +159,54 @@ def precision_errors(m, f):
 
 
 def opacity_grad(args):
-    """Differentiation form: the opacity build and its backward (slgpu_opacity_vjp) at the
-    workload's size, timed with the library's HIP-event phase timers."""
+    """Differentiation form at the workload's size: forward = device EOS (massfr2natomfr +
+    LTE Saha solve) + opacity build + means, backward = opacity VJP + EOS VJP, giving
+    dL/d(T, rho, X) for dL/d(cap, Planck mean, Rosseland mean).  Timed with the library's
+    HIP-event timers: whole phases (including the host transfers of the inputs and of the
+    nc x ng output gradient) and the kernels alone."""
     from superlite_amd import slgpu, synth
     w, st, wl, lines = synth.build(args.workload)
     tables = synth.load_tables()
-    ctx = slgpu.setup_workload(w, st, wl, lines, tables, npartmax=1024, seed=2024)
+    prec = slgpu.SLGPU_MIXED if args.precision == "mixed" else slgpu.SLGPU_FP64
+    ctx = slgpu.setup_workload(w, st, wl, lines, tables, npartmax=1024, seed=2024, precision=prec)
+    ctx.set_eos_data(synth.load_eos_data(args.workload))
     rng = np.random.default_rng(7)
     gcap = rng.standard_normal((w.nx, w.ng))
     ones = np.ones(w.nx)
-    gas = (st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"], st["oc"], st["ginv"])
+    nelec = st["nelec"].copy()
+
+    def step():
+        e = ctx.eos(st["temp"], st["mass"], st["massfr"], nelec=nelec)
+        ctx.opacity_build(st["temp"], e["natom"], st["vol"], st["mass"], e["nelec"], e["natomfr"],
+                          e["oc"], e["ginv"], want=())
+        g = ctx.opacity_vjp(gcap, ones, ones)
+        ctx.eos_vjp(g["natom"], g["natomfr"], g["nelec"], g["oc"])
     for _ in range(args.warmup):
-        ctx.opacity_build(*gas, want=())
-        ctx.opacity_vjp(gcap, ones, ones)
+        step()
     ctx.timers(reset=True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.opacity_build(*gas, want=())
-        ctx.opacity_vjp(gcap, ones, ones)
+    K = min(args.steps, 200)
+    for _ in range(K):
+        step()
     dt = time.perf_counter() - t0
     tm = ctx.timers()
     ctx.close()
-    K = args.steps
     cg = w.nx * w.ng
-    fwd, bwd = tm["opacity"] / K, tm["opacity_vjp"] / K
+    fwd = (tm["eos"] + tm["opacity"]) / K
+    bwd = (tm["opacity_vjp"] + tm["eos_vjp"]) / K
+    fk, bk = tm["opacity_kernels"] / K, tm["opacity_vjp_kernels"] / K
     print(json.dumps({
         "metric": "opacity cell·groups/s, forward and backward (differentiation form)",
         "value": cg / (bwd * 1e-3), "unit": "cell·groups/s (backward)",
         "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": dt / K * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == slgpu.SLGPU_MIXED else "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {w.nx} shells x {w.ng} groups x {w.nline} lines, "
-                               "forward + VJP w.r.t. T, natom, n_e, mass fractions, populations"},
+                               "EOS + opacity forward, VJP w.r.t. T, rho and mass fractions"},
         "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
+        "opacity_kernels_forward_ms": fk, "opacity_kernels_backward_ms": bk,
+        "kernels_backward_over_forward": bk / fk,
+        "eos_ms": tm["eos"] / K, "eos_vjp_ms": tm["eos_vjp"] / K,
         "forward_cell_groups_per_s": cg / (fwd * 1e-3)}))
 
 

@@ -218,7 +218,8 @@ int slgpu_sample_groups(slgpu_ctx *ctx, int table, int direct, int64_t n, const 
 /* accumulated device time per phase in ms, measured with HIP events on the context
  * stream: [0] opacity, [1] ddmc prep, [2] sources, [3] transport, [4] fluxtally,
  * [5] opacity bb+bf+ff kernel alone, [6] transport kernel alone, [7] opacity backward,
- * [8] EOS, [9] EOS backward; n <= 10; reset = 1 zeroes them */
+ * [8] EOS, [9] EOS backward, [10] opacity backward kernels alone, [11] opacity forward
+ * kernels alone (cell prep, bb+bf+ff, transpose, means); n <= 12; reset = 1 zeroes them */
 int slgpu_timers(slgpu_ctx *ctx, double *ms, int n, int reset);
 
 /* the HIP stream all work of the context is ordered on (hipStream_t) */

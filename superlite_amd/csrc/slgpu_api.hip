@@ -196,7 +196,7 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
         delete c;
         return fail(SLGPU_EHIP, "hipStreamCreate failed");
     }
-    for (int k = 0; k < 20; ++k) hipEventCreate(&c->ev[k]);
+    for (int k = 0; k < 24; ++k) hipEventCreate(&c->ev[k]);
     const size_t nc = c->nx, ng = c->ng, cg = nc * ng, np = cfg->npartmax;
     int rc = 0;
     rc |= dalloc(c, (double **)&d.xarr, nc + 1);
@@ -302,7 +302,7 @@ int slgpu_destroy(slgpu_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : c->allocations) hipFree(p);
     c->allocations.clear();
-    for (int k = 0; k < 20; ++k) hipEventDestroy(c->ev[k]);
+    for (int k = 0; k < 24; ++k) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -548,11 +548,13 @@ int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *ou
         HIPCHK(hipMemcpyAsync(doc, gas->oc, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(dgi, gas->ginv, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
     }
+    hipEventRecord(c->ev[22], c->stream);
     launch_cell_prep(c, dn, dv, de, doc, dgi, flags);
     hipEventRecord(c->ev[10], c->stream);
     launch_opacity(c, flags);
     hipEventRecord(c->ev[11], c->stream);
     launch_opacity_finish(c);
+    hipEventRecord(c->ev[23], c->stream);
     HIPCHK(hipGetLastError());
     const size_t cg = (size_t)nc * ng;
     if (out_cap64) HIPCHK(hipMemcpyAsync(out_cap64, c->d.cap64, cg * 8, hipMemcpyDeviceToHost, c->stream));
@@ -565,6 +567,8 @@ int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *ou
     float kms = 0.f;
     hipEventElapsedTime(&kms, c->ev[10], c->ev[11]);
     c->t_ms[5] += kms;
+    hipEventElapsedTime(&kms, c->ev[22], c->ev[23]);
+    c->t_ms[11] += kms;
     c->opac_flags = flags;
     if (rc) return rc;
     c->have_opacity = true;
@@ -785,8 +789,10 @@ int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad
     if (dgp) HIPCHK(hipMemcpyAsync(dgp, grad_capgrey, nc * 8, hipMemcpyHostToDevice, c->stream));
     if (dgr) HIPCHK(hipMemcpyAsync(dgr, grad_capross, nc * 8, hipMemcpyHostToDevice, c->stream));
     double *gT = out, *gn = out + nc, *ge = out + 2 * nc, *gf = out + 3 * nc, *go = out + 33 * (size_t)nc;
+    hipEventRecord(c->ev[20], c->stream);
     launch_opacity_vjp(c, c->opac_flags, dgc, dgp, dgr, gT, gn, ge, gf, go, ff_prefactor(),
                        rgg_scratch(c));
+    hipEventRecord(c->ev[21], c->stream);
     HIPCHK(hipGetLastError());
     if (out_grad_temp) HIPCHK(hipMemcpyAsync(out_grad_temp, gT, nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_grad_natom) HIPCHK(hipMemcpyAsync(out_grad_natom, gn, nc * 8, hipMemcpyDeviceToHost, c->stream));
@@ -794,6 +800,11 @@ int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad
     if (out_grad_natomfr) HIPCHK(hipMemcpyAsync(out_grad_natomfr, gf, (size_t)30 * nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_grad_oc && nion > 0) HIPCHK(hipMemcpyAsync(out_grad_oc, go, (size_t)nion * nc * 8, hipMemcpyDeviceToHost, c->stream));
     rc = tm.stop();
+    if (!rc) {
+        float kms = 0.f;
+        hipEventElapsedTime(&kms, c->ev[20], c->ev[21]);
+        c->t_ms[10] += kms;
+    }
     dfree(c, dgc); dfree(c, dgp); dfree(c, dgr); dfree(c, out);
     return rc;
 }
@@ -930,7 +941,7 @@ int slgpu_eos_vjp(slgpu_ctx *c, const double *grad_natom, const double *grad_nat
 int slgpu_timers(slgpu_ctx *c, double *ms, int n, int reset)
 {
     if (!c) return fail(SLGPU_EINVAL, "null context");
-    for (int k = 0; k < n && k < 10; ++k) ms[k] = c->t_ms[k];
+    for (int k = 0; k < n && k < 12; ++k) ms[k] = c->t_ms[k];
     if (reset)
         for (double &t : c->t_ms) t = 0.0;
     return 0;

@@ -135,8 +135,8 @@ struct slgpu_ctx {
            *eos_natom = nullptr, *eos_natomfr = nullptr, *eos_oc = nullptr, *eos_ginv = nullptr;
     int *eos_niter = nullptr;
     void *scratch = nullptr;    // device address of the library's static scratch array on this context's GPU
-    double t_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev[20];
+    double t_ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[24];
 };
 
 namespace slgpu {

@@ -285,11 +285,11 @@ class Context:
         return out
 
     def timers(self, reset=False):
-        ms = np.zeros(10)
-        self._check(self.L.slgpu_timers(self.h, _p(ms), 10, int(reset)))
+        ms = np.zeros(12)
+        self._check(self.L.slgpu_timers(self.h, _p(ms), 12, int(reset)))
         return dict(opacity=ms[0], prep=ms[1], sources=ms[2], transport=ms[3], fluxtally=ms[4],
                     opacity_kernel=ms[5], transport_kernel=ms[6], opacity_vjp=ms[7], eos=ms[8],
-                    eos_vjp=ms[9])
+                    eos_vjp=ms[9], opacity_vjp_kernels=ms[10], opacity_kernels=ms[11])
 
 
 def peak_rates():
