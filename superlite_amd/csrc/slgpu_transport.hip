@@ -143,13 +143,19 @@ template <bool M> __device__ __forceinline__ double p_adiabatic(double divv, dou
     }
 }
 // r^3-uniform radius in [xl, xr]: (r*xr^3 + (1-r)*xl^3)**(1.0/3.0) with the reference's
-// single-precision exponent (diffusion11.f90:336,597).  Radii cubed (~1e47 cm^3) overflow
-// FP32, so the precision form takes the root of the normalised r + (1-r)*(xl/xr)^3.
+// single-precision exponent e = (1.0/3.0 in single) = (1 + 2^-25)/3 (diffusion11.f90:336,597).
+// Radii cubed (~1e47 cm^3) overflow FP32, so the precision form takes the FP32 root of the
+// normalised b = r + (1-r)*(xl/xr)^3 and multiplies by xr^(3e) = xr * exp(2^-25 * ln xr).
+// That last factor (~1 + 1.1e-6 at supernova radii) is what carries the outermost draws past
+// xr into the caller's clamp, as in the FP64 form; dropping it moves ~0.1% of the
+// conversions in thin shells off the face and biased the step count by +0.75%.
 template <bool M> __device__ __forceinline__ double p_shell_root(double r1, double xl, double xr)
 {
     if constexpr (M) {
         const double q = xl / xr;
-        return xr * (double)powf((float)(r1 + (1.0 - r1) * (q * q * q)), 1.0f / 3.0f);
+        const double dl = 0x1p-25 * (double)__logf((float)xr);
+        const double xr3e = xr * (1.0 + dl * (1.0 + 0.5 * dl));
+        return xr3e * (double)powf((float)(r1 + (1.0 - r1) * (q * q * q)), 1.0f / 3.0f);
     } else {
         return pow(r1 * (xr * xr * xr) + (1.0 - r1) * (xl * xl * xl), (double)(1.0f / 3.0f));
     }
