@@ -630,32 +630,42 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     return ret;
 }
 
-struct Accum {
-    double evelo, eout, edead, estart, esrc, eamp;
-    uint32_t nact, nimc, nddmc, nswap, nflux, ndead, nerr, nwarn, nmax;
+// Block totals in LDS, indexed like Dev::tot and Dev::ct.  Whatever changes once per
+// particle (its start, its end, the rare warnings and amplification energies) is added
+// here at once; only the per-step sums stay in registers (evelo, nddmc, nswap), which
+// keeps the persistent lane state small enough to avoid spilling.  The IMC step count is
+// the particles' total steps minus the DDMC steps.
+struct BlockTotals {
+    double tot[7];                 // eout, evelo, -, estart, edead, esrc, eamp
+    unsigned long long ct[9];      // nact, total steps (then nimc), nddmc, nmax, nswap,
+                                   // nflux, ndead, nerr, nwarn
 };
 
-__device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
+__device__ __forceinline__ void flush_totals(const Dev &d, BlockTotals &b, double evelo,
+                                             uint32_t nddmc, uint32_t nswap)
 {
-    const double vals[6] = {a.eout, a.evelo, a.estart, a.edead, a.esrc, a.eamp};
-    const int slot[6] = {0, 1, 3, 4, 5, 6};
-    for (int q = 0; q < 6; ++q) {
-        const double v = wave_sum(vals[q]);
-        if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&d.tot[slot[q]], v);
-    }
-    const uint32_t cv[8] = {a.nact, a.nimc, a.nddmc, a.nswap, a.nflux, a.ndead, a.nerr, a.nwarn};
-    const int cslot[8] = {0, 1, 2, 4, 5, 6, 7, 8};
-    for (int q = 0; q < 8; ++q) {
-        unsigned long long v = cv[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&d.ct[cslot[q]], v);
-    }
-    unsigned long long v = a.nmax;
+    const double ev = wave_sum(evelo);
+    unsigned long long nd = nddmc, ns = nswap;
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
+        nd += __shfl_xor(nd, off, 64);
+        ns += __shfl_xor(ns, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(&d.ct[3], v);
+    if ((threadIdx.x & 63) == 0) {
+        if (ev != 0.0) atomicAdd(&b.tot[1], ev);
+        atomicAdd(&b.ct[2], nd);
+        atomicAdd(&b.ct[4], ns);
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const double v = b.tot[threadIdx.x];
+        if (v != 0.0) atomicAdd(&d.tot[threadIdx.x], v);
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + 9) {
+        const int q = threadIdx.x - 64;
+        unsigned long long v = b.ct[q];
+        if (q == 1) v -= b.ct[2];                                // IMC steps
+        if (q == 3) atomicMax(&d.ct[3], v);
+        else if (v) atomicAdd(&d.ct[q], v);
+    }
 }
 
 // particle_advance.f90:140-294 as a persistent kernel: each lane takes the next particle
@@ -675,15 +685,26 @@ __device__ __forceinline__ void flush_accum(const Dev &d, const Accum &a)
 #ifndef SLGPU_VOTE_DDMC
 #define SLGPU_VOTE_DDMC 1
 #endif
+#ifndef SLGPU_WAVES_MIXED
+#define SLGPU_WAVES_MIXED 4
+#endif
+#ifndef SLGPU_WAVES_FP64
+#define SLGPU_WAVES_FP64 3
+#endif
 template <bool M>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M ? 4 : 3, 8)))
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(M ? SLGPU_WAVES_MIXED : SLGPU_WAVES_FP64, 8)))
 void k_transport(Dev d, uint32_t iteration)
 {
     extern __shared__ double s_tally[];
     const int ng = d.ng, nx = d.nx;
+    __shared__ BlockTotals bt;
     for (int k = threadIdx.x; k < nx; k += blockDim.x) s_tally[k] = 0.0;
+    if (threadIdx.x < 7) bt.tot[threadIdx.x] = 0.0;
+    if (threadIdx.x < 9) bt.ct[threadIdx.x] = 0;
     __syncthreads();
-    Accum a = {};
+    double evelo = 0.0;
+    uint32_t nddmc = 0, nswap = 0;
     int run_ic = -1, run_ig = -1;
     double run_e = 0.0, run_j = 0.0;
     double *jrad = d.jrad + (size_t)(blockIdx.x % d.jrad_copies) * nx * ng;
@@ -696,12 +717,13 @@ void k_transport(Dev d, uint32_t iteration)
             ip = (int64_t)atomicAdd(d.queue, 1ull);
             if (ip >= d.npart) break;
             p.x = d.px[ip]; p.mu = d.pmu[ip]; p.e = d.pe[ip]; p.e0 = d.pe0[ip]; p.wl = d.pwl[ip];
-            a.esrc += p.e;
-            a.nact++;
+            atomicAdd(&bt.tot[5], p.e);
+            atomicAdd(&bt.ct[0], 1ull);
             p.ix = binsrch(p.x, d.xarr, nx + 1, false);
             p.ig = binsrch(p.wl, d.wl, ng + 1, false);
             if (p.ix < 1 || p.ig < 1) {                        // NaN position or wavelength
-                a.nerr++; a.ndead++; a.edead += p.e;
+                atomicAdd(&bt.ct[7], 1ull); atomicAdd(&bt.ct[6], 1ull);
+                atomicAdd(&bt.tot[4], p.e);
                 d.pstat[ip] = 2; d.perr[ip] = 1; d.pnstep[ip] = 0;
                 continue;
             }
@@ -715,7 +737,7 @@ void k_transport(Dev d, uint32_t iteration)
                 p.e = p.e / labfact;
                 p.e0 = p.e0 / labfact;
             }
-            a.estart += p.e;
+            atomicAdd(&bt.tot[3], p.e);
             p.stat = 0;
             p.idist = 0;
             s.init(d.seed, iteration, (uint32_t)ip, 1u);
@@ -733,18 +755,17 @@ void k_transport(Dev d, uint32_t iteration)
         double erd, jrd;
         int ierr;
         if (imc) {
-            a.nimc++;
             double ea;
-            ierr = transport11<M>(d, p, s, erd, jrd, ea, a.evelo);
+            ierr = transport11<M>(d, p, s, erd, jrd, ea, evelo);
             if (ea != 0.0) {                                   // particle_advance.f90:216-217
                 atomicAdd(&d.eamp[icold - 1], ea);
-                a.eamp += ea;
+                atomicAdd(&bt.tot[6], ea);
             }
-            if (p.itype != 1) a.nswap++;
+            if (p.itype != 1) nswap++;
         } else {
-            a.nddmc++;
-            ierr = diffusion11<M>(d, p, s, erd, jrd, a.evelo);
-            if (p.itype == 1) a.nswap++;
+            nddmc++;
+            ierr = diffusion11<M>(d, p, s, erd, jrd, evelo);
+            if (p.itype == 1) nswap++;
         }
         if (icold != run_ic || igold != run_ig) {
             if (run_ic > 0) {
@@ -760,13 +781,14 @@ void k_transport(Dev d, uint32_t iteration)
                 if (ierr == 0) ierr = -99;
             }
         }
-        if (ierr < 0) a.nwarn++;
+        if (ierr < 0) atomicAdd(&bt.ct[8], 1ull);
         if (ierr == 0 && p.stat == 0 && istep >= d.max_steps) ierr = 200;
-        if (ierr > 0) { a.nerr++; p.stat = 2; }
+        if (ierr > 0) { atomicAdd(&bt.ct[7], 1ull); p.stat = 2; }
         if (p.stat != 0) {
-            if ((uint32_t)istep > a.nmax) a.nmax = (uint32_t)istep;
-            if (p.stat == 2) { a.ndead++; a.edead += p.e; }
-            else { a.nflux++; a.eout += p.e; }
+            atomicMax(&bt.ct[3], (unsigned long long)istep);
+            atomicAdd(&bt.ct[1], (unsigned long long)istep);
+            if (p.stat == 2) { atomicAdd(&bt.ct[6], 1ull); atomicAdd(&bt.tot[4], p.e); }
+            else { atomicAdd(&bt.ct[5], 1ull); atomicAdd(&bt.tot[0], p.e); }
             d.pstat[ip] = (uint8_t)p.stat;
             d.px[ip] = p.x; d.pmu[ip] = p.mu; d.pwl[ip] = p.wl; d.pe[ip] = p.e; d.pe0[ip] = p.e0;
             d.pig[ip] = p.ig; d.pix[ip] = p.ix;
@@ -782,7 +804,7 @@ void k_transport(Dev d, uint32_t iteration)
     __syncthreads();
     for (int k = threadIdx.x; k < nx; k += blockDim.x)
         if (s_tally[k] != 0.0) atomicAdd(&d.tally[k], s_tally[k]);
-    flush_accum(d, a);
+    flush_totals(d, bt, evelo, nddmc, nswap);
 }
 
 // fluxtally.f:30-54 (flux grid == group grid; polar/azimuth bins are single)
