@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_bucket_tables(Dev d)
     uint32_t e = 0;
     if (t == TAB_E) {
         const int glo = emitgroup_direct(d, rlo, ic), ghi = emitgroup_direct(d, rhi, ic);
-        if (glo == ghi) e = BT_PURE | (dest_thin(d, t, ic, glo) ? BT_THIN : 0u) | (uint32_t)glo;
+        if (glo == ghi) e = BT_PURE | dest_flags(d, t, ic, glo) | (uint32_t)glo;
         else e = ((uint32_t)ghi << 15) | (uint32_t)glo;
     } else {
         const LumpRow L = lump_row(d, t, ic, d.cell[ic - 1].nlump);
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void k_bucket_tables(Dev d)
             const int plo = jlo < L.cnt ? jlo : L.cnt - 1, phi = jhi < L.cnt ? jhi : L.cnt - 1;
             if (plo == phi) {
                 const int g = lump_group(d, L, ic, jlo);
-                e = BT_PURE | (dest_thin(d, t, ic, g) ? BT_THIN : 0u) | (uint32_t)g;
+                e = BT_PURE | dest_flags(d, t, ic, g) | (uint32_t)g;
             } else {
                 e = ((uint32_t)jhi << 15) | (uint32_t)jlo;
             }

@@ -146,23 +146,26 @@ struct Philox {
                                                  uint32_t out[4])
     {
         // rolled: one copy of the round per call site keeps the transport kernel's code
-        // small enough for the instruction cache
+        // small enough for the instruction cache.  Each 32x32 product is formed as one
+        // 64-bit product (one v_mad_u64_u32) rather than separate low and high halves
+        // (v_mul_lo_u32 + v_mul_hi_u32): half the quarter-rate multiplies, same bits.
         #pragma unroll 1
         for (int r = 0; r < 10; ++r) {
-            uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-            uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-            uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-            c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+            const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+            const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+            c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
         out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
     }
 
+    // (m + 0.5) * 2^-53 for the 53-bit m = (a >> 5) * 2^26 + (b >> 6).  The fma rounds the
+    // exact m + 0.5 once, as (double)m + 0.5 does (m itself is exact), without the
+    // 64-bit-integer-to-double sequence.
     __device__ __forceinline__ static double u01(uint32_t a, uint32_t b)
     {
-        uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
-        return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+        return fma((double)(a >> 5), 67108864.0, (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
     }
 
     __device__ __forceinline__ void fill(uint32_t j)

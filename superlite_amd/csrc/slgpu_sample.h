@@ -14,8 +14,11 @@
 // capped at 1, where the scatter event itself has probability zero up to rounding.
 //
 // Entry layout (uint32): bit 31 pure; pure: bit 30 = destination optically thin (the
-// particle leaves DDMC for IMC), bits 0-14 = group; impure: bits 15-29 = hi, 0-14 = lo
+// particle leaves DDMC for IMC), bit 29 = destination group lumped (the next DDMC step
+// uses the cell's lumped thresholds), bits 0-14 = group; impure: bits 15-29 = hi, 0-14 = lo
 // (group numbers for emitgroup, row positions for the lumped tables).  ng <= 32767.
+// The two destination bits are what the particle's next step would otherwise load the
+// group opacity for, so a pure entry hands that step its thresholds' address directly.
 #pragma once
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
@@ -24,7 +27,7 @@ namespace slgpu {
 
 constexpr int BT_BITS = 14;
 constexpr int BT_K = 1 << BT_BITS;
-constexpr uint32_t BT_PURE = 0x80000000u, BT_THIN = 0x40000000u;
+constexpr uint32_t BT_PURE = 0x80000000u, BT_THIN = 0x40000000u, BT_LUMP = 0x20000000u;
 enum { TAB_L = 0, TAB_R = 1, TAB_D = 2, TAB_S = 3, TAB_E = 4, NTAB = 5 };
 
 __device__ __forceinline__ int bt_key(double r)
@@ -56,6 +59,23 @@ __device__ __forceinline__ bool is_thin(const Dev &d, int g, int lc)
 {
     const double dx = d.xarr[lc] - d.xarr[lc - 1];
     return d.puretran || (d.sig[lc - 1] + cap_of(d, g, lc)) * dx < d.tauddmc;
+}
+
+// BT_LUMP when a DDMC step of group g in cell lc runs lumped: glump = nlump > 0 exactly when
+// cap*dx >= taulump (diffusion11.f90:156-160) and the cell has lumped groups at all
+// (nlump = 0 when speclump = 0, :107-110).  opacity: the group's cap as a double.
+__device__ __forceinline__ uint32_t lump_flag(const Dev &d, double opacity, double dx, int lc)
+{
+    return (opacity * dx >= d.taulump && d.cell[lc - 1].nlump > 0) ? BT_LUMP : 0u;
+}
+
+// BT_THIN and BT_LUMP of group g in cell lc from one opacity load
+__device__ __forceinline__ uint32_t group_flags(const Dev &d, int g, int lc)
+{
+    const double dx = d.xarr[lc] - d.xarr[lc - 1];
+    const double cp = cap_of(d, g, lc);
+    if (d.puretran || (d.sig[lc - 1] + cp) * dx < d.tauddmc) return BT_THIN;
+    return lump_flag(d, cp, dx, lc);
 }
 
 // first index j in [lo, hi) with cum[j] > r (upper bound); hi if none
@@ -131,8 +151,9 @@ __device__ __forceinline__ int lump_group(const Dev &d, const LumpRow &L, int ic
     return d.gorder[(size_t)(ic - 1) * d.ng + pos];
 }
 
-// the group a table's event moves the particle to, and the cell that receives it
-__device__ __forceinline__ bool dest_thin(const Dev &d, int t, int ic, int iiig)
+// BT_THIN / BT_LUMP of the group a table's event moves the particle to, in the cell that
+// receives it (0 past the outer face: the particle escapes instead)
+__device__ __forceinline__ uint32_t dest_flags(const Dev &d, int t, int ic, int iiig)
 {
     int lc = ic, g = iiig;
     if (t == TAB_L) lc = ic - 1;
@@ -141,27 +162,37 @@ __device__ __forceinline__ bool dest_thin(const Dev &d, int t, int ic, int iiig)
         if (d.divv[ic - 1] >= 0) g = iiig + 1 > d.ng ? d.ng : iiig + 1;
         else g = iiig - 1 < 1 ? 1 : iiig - 1;
     }
-    if (lc < 1 || lc > d.nx) return false;     // outer face: the particle escapes instead
-    return is_thin(d, g, lc);
+    if (lc < 1 || lc > d.nx) return 0u;
+    return group_flags(d, g, lc);
 }
 
-// sampled group of table t in cell ic for draw r; `thin` is set to -1 when the caller
-// must evaluate the destination test itself (impure bucket), else 0/1
-__device__ __forceinline__ int sample_group(const Dev &d, int t, int ic, int nlump, double r,
-                                            int &thin)
+// sampled group of table t in cell ic for draw r, given the draw's bucket entry e of that
+// table; fl = the destination's BT_THIN / BT_LUMP bits of a pure entry, or -1 when the
+// caller must evaluate them itself (impure bucket)
+__device__ __forceinline__ int sample_entry(const Dev &d, int t, int ic, int nlump, double r,
+                                            uint32_t e, int &fl)
 {
-    const uint32_t e = bt_row(d, t, ic)[bt_key(r)];
     if (e & BT_PURE) {
-        thin = (e & BT_THIN) ? 1 : 0;
+        fl = (int)(e & (BT_THIN | BT_LUMP));
         return (int)(e & 0x7fffu);
     }
-    thin = -1;
+    fl = -1;
     const int lo = (int)(e & 0x7fffu), hi = (int)((e >> 15) & 0x7fffu);
     if (t == TAB_E) return emitgroup_range(d, r, ic, lo, hi);
     const LumpRow L = lump_row(d, t, ic, nlump);
     int j = lo;
     while (j < hi && !(L.cum[j] > r)) ++j;
     return lump_group(d, L, ic, j);
+}
+
+// the same with the entry loaded here; `thin` is -1 (impure bucket: not evaluated) or 0/1
+__device__ __forceinline__ int sample_group(const Dev &d, int t, int ic, int nlump, double r,
+                                            int &thin)
+{
+    int fl;
+    const int g = sample_entry(d, t, ic, nlump, r, bt_row(d, t, ic)[bt_key(r)], fl);
+    thin = fl < 0 ? -1 : ((fl & (int)BT_THIN) ? 1 : 0);
+    return g;
 }
 
 }  // namespace slgpu

@@ -107,7 +107,11 @@ __global__ __launch_bounds__(256) void k_sources(Dev d, uint32_t iteration)
 struct Ptcl {
     double x, mu, e, e0, wl, vx;
     int ix, ig, itype, stat, idist;
+    // for a DDMC particle: NF_VALID | BT_LUMP of its current (group, cell) when the step that
+    // put it there knew them, so its next step needs no opacity load to pick its thresholds
+    uint32_t nflag;
 };
+constexpr uint32_t NF_VALID = 1u;
 
 // TRANSPORT1/transport11.f90:86-489 (trn_noampfact = .true.)
 // Arithmetic of the precision form.  SLGPU_MIXED evaluates the sampled path lengths and
@@ -288,10 +292,13 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         e0 = e0 * help;
         if (ng != 1) {
             r1 = s.draw();
-            int thin;
-            ig = sample_group(d, TAB_E, ic, 0, r1, thin);
-            if (thin < 0) thin = is_thin(d, ig, ic);
-            if (!thin) p.itype = 2;
+            int fl;
+            ig = sample_entry(d, TAB_E, ic, 0, r1, bt_row(d, TAB_E, ic)[bt_key(r1)], fl);
+            if (fl < 0) fl = (int)group_flags(d, ig, ic);
+            if (!(fl & (int)BT_THIN)) {
+                p.itype = 2;
+                p.nflag = NF_VALID | ((uint32_t)fl & BT_LUMP);
+            }
         }
         if (p.itype == 2) {
             evelo = evelo + e * vx * mu * CINV;
@@ -320,6 +327,7 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             r1 = s.draw();
             if (r1 < help * (1.0 + 1.5 * fabs(mu))) {
                 p.itype = 2;
+                p.nflag = NF_VALID | lump_flag(d, CAPT(ig, l), DXX(ix + 1), l);
                 evelo = evelo + e * (1.0 - elabfact);
                 e = e * elabfact;
                 e0 = e0 * elabfact;
@@ -357,6 +365,7 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             r1 = s.draw();
             if (r1 < help * (1.0 + 1.5 * fabs(mu))) {
                 p.itype = 2;
+                p.nflag = NF_VALID | lump_flag(d, CAPT(ig, l), DXX(ix - 1), l);
                 evelo = evelo + e * (1.0 - elabfact);
                 e = e * elabfact;
                 e0 = e0 * elabfact;
@@ -393,6 +402,7 @@ __device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         }
         if (((SIGT(ic) + CAPT(ig, ic)) * DXX(ix) >= d.tauddmc) && !d.puretran) {
             p.itype = 2;
+            p.nflag = NF_VALID | lump_flag(d, CAPT(ig, ic), DXX(ix), ic);
             evelo = evelo + e * vx * mu * CINV;
             e = e * elabfact;
             e0 = e0 * elabfact;
@@ -512,27 +522,48 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
     erd = 0.0; jrd = 0.0;
     const double dist = DXX(ix);
     const int nlump = d.cell[ic - 1].nlump;
-    const double cp = CAPT(ig, ic);
-    const int glump = (cp * dist >= d.taulump) ? nlump : 0;
-    if ((SIGT(ic) + cp) * dist < d.tauddmc) return 100;
-    // one load path for both kinds of group: the cell's lumped thresholds or the group's own
-    const DdmcProb *qp = glump > 0 ? d.cprob + (ic - 1) : d.gprob + (size_t)(ic - 1) * ng + (ig - 1);
-    const DdmcProb q = *qp;
+    // The step's first three draws.  Draw 2 samples the group of every effective scatter
+    // (and of lumped events), so with the flags the previous step left, the thresholds and
+    // that draw's bucket entry are requested together: one memory round trip per step
+    // instead of opacity -> thresholds -> entry.  Draws are counted on (step, index), so
+    // drawing early does not change any value.
+    const double rt = s.draw(), rev = s.draw(), r2 = s.draw();
+    const bool known = (p.nflag & NF_VALID) != 0;
+    uint32_t lump;                                             // BT_LUMP of (ig, ic)
+    if (known) {
+        lump = p.nflag & BT_LUMP;
+    } else {
+        const double cp = CAPT(ig, ic);
+        if ((SIGT(ic) + cp) * dist < d.tauddmc) return 100;
+        lump = (cp * dist >= d.taulump && nlump > 0) ? BT_LUMP : 0u;
+    }
+    const int glump = lump ? nlump : 0;                        // :156-160
+    // the cell's lumped thresholds or the group's own
+    const DdmcProb q = lump ? d.cprob[ic - 1] : d.gprob[(size_t)(ic - 1) * ng + (ig - 1)];
+    const int tscat = lump ? TAB_S : TAB_E;
+    const uint32_t escat = known ? bt_row(d, tscat, ic)[bt_key(r2)] : 0u;
     const double denom = q.denom;
-    r1 = s.draw();
-    const double ddmct = fabs(p_log<M>(r1) * denom * CINV);
+    const double ddmct = fabs(p_log<M>(rt) * denom * CINV);
     erd = e * ddmct;
     jrd = PC_C * erd;
-    r1 = s.draw();
     int ev;
-    if (r1 >= 0.0 && r1 < q.t1) ev = EV_DOP;
-    else if (r1 >= q.t1 && r1 < q.t2) ev = EV_LEFT;
-    else if (r1 >= q.t2 && r1 < q.t3) ev = EV_RIGHT;
+    if (rev >= 0.0 && rev < q.t1) ev = EV_DOP;
+    else if (rev >= q.t1 && rev < q.t2) ev = EV_LEFT;
+    else if (rev >= q.t2 && rev < q.t3) ev = EV_RIGHT;
     else ev = EV_SCAT;
+    // keep the entry's load ahead of the branches that use it
+    asm volatile("" ::"v"(escat));
     p.idist = ev == EV_LEFT ? -3 : (ev == EV_RIGHT ? -4 : (ev == EV_SCAT ? -2 : p.idist));
-    const size_t row = (size_t)(ic - 1) * ng;
+    // draws after the first three continue the step's sequence; draw 2 is taken by the
+    // first consumer that needs a fourth number in the reference's order
+    bool r2_free = true;
+    auto next = [&]() -> double {
+        if (r2_free) { r2_free = false; return r2; }
+        return s.draw();
+    };
     int ret = 0;
     bool adiabatic = true;
+    uint32_t nflag = NF_VALID | lump;                          // particle stays in (ig, ic)
     if (ev == EV_LEFT && ix == 1) {
         adiabatic = false;                                     // :362-365 returns
     } else if (ev == EV_SCAT && glump == ng) {
@@ -542,21 +573,23 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
         const bool flux = (ev == EV_RIGHT && ix == nx);        // :438-478
         if (flux) {
             p.stat = 1;
-            const double ra = s.draw(), rb = s.draw();
+            const double ra = next(), rb = next();
             mu = ra > rb ? ra : rb;
         }
         // group sampling: lumped particles draw from the cell's table of this event,
         // unlumped effective scatters from the emission probabilities (emitgroup)
-        int iiig = ig, thin = -1;
+        int iiig = ig, fl = -1;
         if (glump > 0 || ev == EV_SCAT) {
-            r1 = s.draw();
+            const bool pre = known && r2_free && ev == EV_SCAT;
+            r1 = next();
             const int t = glump == 0 ? TAB_E
                         : (ev == EV_LEFT ? TAB_L : (ev == EV_RIGHT ? TAB_R : (ev == EV_DOP ? TAB_D : TAB_S)));
-            iiig = sample_group(d, t, ic, nlump, r1, thin);
+            const uint32_t ent = pre ? escat : bt_row(d, t, ic)[bt_key(r1)];
+            iiig = sample_entry(d, t, ic, nlump, r1, ent, fl);
         }
         if (flux) {
             ig = iiig;
-            r1 = s.draw();
+            r1 = next();
             wl = 1.0 / (r1 * WLI(ig + 1) + (1.0 - r1) * WLI(ig));
             x = XA(nx + 1);
             vx = VXA(nx + 1);
@@ -579,19 +612,19 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
             } else if (ev == EV_RIGHT) {
                 lc = ix + 1;
             }
-            if (thin < 0) thin = is_thin(d, ig2, lc);
-            if (thin) {
+            if (fl < 0) fl = (int)group_flags(d, ig2, lc);
+            if (fl & (int)BT_THIN) {
                 // conversion to IMC (:329-355, 402-423, 515-538, 587-618)
                 p.itype = 1;
                 const bool face = (ev == EV_LEFT || ev == EV_RIGHT);
                 if (ev == EV_SCAT) {
-                    r1 = s.draw();
+                    r1 = next();
                     wl = 1.0 / ((1.0 - r1) * WLI(ig2) + r1 * WLI(ig2 + 1));
                 } else if (face) {
-                    r1 = s.draw();
+                    r1 = next();
                     wl = 1.0 / (r1 * WLI(iiig + 1) + (1.0 - r1) * WLI(iiig));
                 }
-                const double ra = s.draw(), rb = s.draw();
+                const double ra = next(), rb = next();
                 if (face) {
                     const int xi = ev == EV_LEFT ? ix : ix + 1;
                     x = XA(xi);
@@ -610,11 +643,13 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double
                 e0 = e0 * help;
                 wl = wl * (1.0 - vx * mu * CINV);
             }
+            nflag = NF_VALID | ((uint32_t)fl & BT_LUMP);
             ig = ig2;
             if (ev == EV_LEFT) ix = ix - 1;
             else if (ev == EV_RIGHT) ix = ix + 1;
         }
     }
+    p.nflag = nflag;
     if (adiabatic) {
         if (x < XA(1)) {
             p.stat = 2;                                        // :623-626
@@ -740,6 +775,7 @@ void k_transport(Dev d, uint32_t iteration)
             atomicAdd(&bt.tot[3], p.e);
             p.stat = 0;
             p.idist = 0;
+            p.nflag = 0;
             s.init(d.seed, iteration, (uint32_t)ip, 1u);
             istep = 0;
             have = true;
