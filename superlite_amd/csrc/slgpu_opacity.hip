@@ -9,6 +9,7 @@
 // the precomputed per-group line offsets, so every sum runs in the reference's line order
 // and no cross-lane combine is needed.  bf and ff are accumulated into the same register
 // afterwards, in the reference's order (bb, then bf by ion, then ff by element).
+#include <type_traits>
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
 
@@ -117,15 +118,57 @@ __global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const doubl
     }
 }
 
+// exp(-x) for x >= 0 in FP32 for the precision form's line terms: 2^(-x log2 e) with the
+// rounding error of the product -x*log2(e) (its fma residual plus the tail of log2 e)
+// applied to first order.  About 1.5 ulp, against ~1 ulp for expf at half the instructions.
+__device__ __forceinline__ float exp_neg32(float x)
+{
+    const float y = -x * 1.44269502f;
+    const float e = fmaf(-x, 1.44269502f, -y) + -x * 1.925963e-8f;
+    const float p = __builtin_amdgcn_exp2f(y);
+    return fmaf(p, e * 0.693147181f, p);
+}
+
+// order LDS writes of a wave before its later reads (each wave owns its slot)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Fused bb + bf + ff for one (cell, group); result into capT[g*nc + i] (cell-fastest).
 // blockDim = 256 = 64 cells x 4 groups; grid = (ceil(nc/64), ceil(ng/4)).
-template <bool M>
+// S: the block's 64 cells' line populations oc*ginv of every ion are staged in LDS once
+// (FP32 in the precision form, whose line terms are FP32), so each line costs one LDS read
+// per lane instead of a global gather of the 64 cells' populations.  A population that is
+// not positive is staged as 0: its term is then exactly 0, as the reference's skip of
+// such ions gives, and the line loop needs no test.
+template <bool M, bool S>
 __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
                                                   const double *__restrict__ rgg_tab)
 {
+    using PT = typename std::conditional<M, float, double>::type;
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    PT *s_oc = reinterpret_cast<PT *>(s_raw);
+    // S: each wave also copies its group's line records into LDS 64 at a time (one
+    // coalesced vector load), so the line loop holds only LDS reads, which complete in
+    // order and pipeline, instead of a scalar load it must wait for on every line
+    __shared__ LineRec s_lines[4][64];
     const int nc = d.nx, ng = d.ng;
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 64 + lane;
+    if constexpr (S) {
+        if (!(flags & SLGPU_NOBB)) {
+            const int c0 = blockIdx.x * 64;
+            for (int k = threadIdx.x; k < d.nion * 64; k += 256) {
+                const int cc = c0 + (k & 63);
+                const double o = cc < nc ? d.ocg[(size_t)(k >> 6) * nc + cc] : 0.0;
+                s_oc[k] = o > 0.0 ? (PT)o : (PT)0;
+            }
+            __syncthreads();
+        }
+    }
     // wave-uniform group index (1-based)
     const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6)) + 1;
     if (g > ng) return;
@@ -149,22 +192,65 @@ __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
             // precision form: FP32 terms, Kahan-compensated FP32 sum, widened once
             const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
             float sum = 0.0f, comp = 0.0f;
-            for (int l = l0; l < l1; ++l) {
-                const LineRec r = d.lines[l];
-                const double o = ocg[(size_t)r.ion * nc];
-                const float t = phi32 * r.gxs * (float)o * expf(-(r.chilw * hk32)) * ef32;
-                const float y = ((o > 0.0) ? t : 0.0f) - comp;
-                const float u = sum + y;
-                comp = (u - sum) - y;
-                sum = u;
+            if constexpr (S) {
+                LineRec *ln = s_lines[threadIdx.x >> 6];
+                for (int base = l0; base < l1; base += 64) {
+                    const int n = l1 - base < 64 ? l1 - base : 64;
+                    wave_lds_sync();
+                    if (lane < n) {
+                        LineRec r = d.lines[base + lane];
+                        r.gxs = phi32 * r.gxs;                 // the term's first product
+                        ln[lane] = r;
+                    }
+                    wave_lds_sync();
+                    #pragma unroll 4
+                    for (int j = 0; j < n; ++j) {
+                        const LineRec r = ln[j];
+                        const float of = s_oc[r.ion * 64 + lane];
+                        const float t = r.gxs * of * exp_neg32(r.chilw * hk32) * ef32;
+                        const float y = t - comp;
+                        const float u = sum + y;
+                        comp = (u - sum) - y;
+                        sum = u;
+                    }
+                }
+            } else {
+                for (int l = l0; l < l1; ++l) {
+                    const LineRec r = d.lines[l];
+                    // (float)o > 0 exactly when o > 0 gives a nonzero term
+                    const float of = (float)ocg[(size_t)r.ion * nc];
+                    const float t = phi32 * r.gxs * of * exp_neg32(r.chilw * hk32) * ef32;
+                    const float y = ((of > 0.0f) ? t : 0.0f) - comp;
+                    const float u = sum + y;
+                    comp = (u - sum) - y;
+                    sum = u;
+                }
             }
             acc += (double)sum;
         } else {
-            for (int l = l0; l < l1; ++l) {
-                const LineRec r = d.lines[l];             // uniform: scalar loads
-                const double o = ocg[(size_t)r.ion * nc];
-                const double term = phi * (double)r.gxs * o * exp(-((double)r.chilw * hckt)) * expfac;
-                acc += (o > 0.0) ? term : 0.0;
+            if constexpr (S) {
+                LineRec *ln = s_lines[threadIdx.x >> 6];
+                for (int base = l0; base < l1; base += 64) {
+                    const int n = l1 - base < 64 ? l1 - base : 64;
+                    wave_lds_sync();
+                    if (lane < n) ln[lane] = d.lines[base + lane];
+                    wave_lds_sync();
+                    #pragma unroll 2
+                    for (int j = 0; j < n; ++j) {
+                        const LineRec r = ln[j];
+                        const double o = s_oc[r.ion * 64 + lane];
+                        // the select keeps the product and the sum separately rounded
+                        const double term = phi * (double)r.gxs * o * exp(-((double)r.chilw * hckt)) * expfac;
+                        acc += (o > 0.0) ? term : 0.0;
+                    }
+                }
+            } else {
+                for (int l = l0; l < l1; ++l) {
+                    const LineRec r = d.lines[l];         // uniform: scalar loads
+                    const double o = ocg[(size_t)r.ion * nc];
+                    const double term = phi * (double)r.gxs * o * exp(-((double)r.chilw * hckt)) * expfac;
+                    acc += (o > 0.0) ? term : 0.0;
+                }
             }
         }
     }
@@ -309,12 +395,24 @@ void launch_opacity(slgpu_ctx *c, int flags)
 {
     const double ffc1 = ff_prefactor();
     dim3 grid((c->nx + 63) / 64, (c->ng + 3) / 4);
-    if (c->d.precision == SLGPU_MIXED)
-        hipLaunchKernelGGL(k_opacity<true>, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,
-                           (const double *)g_rgg_scratch(c));
+    const double *rgg = g_rgg_scratch(c);
+    const bool mixed = c->d.precision == SLGPU_MIXED;
+    // staged populations: nion x 64 cells, up to 64 KiB of LDS per block beside the 4 KiB
+    // of line records
+    const size_t lds = (size_t)c->nion * 64 * (mixed ? sizeof(float) : sizeof(double));
+    const bool staged = c->nion > 0 && lds <= 65536;
+    if (mixed && staged)
+        hipLaunchKernelGGL((k_opacity<true, true>), grid, dim3(256), lds, c->stream, c->d, flags,
+                           ffc1, rgg);
+    else if (mixed)
+        hipLaunchKernelGGL((k_opacity<true, false>), grid, dim3(256), 0, c->stream, c->d, flags,
+                           ffc1, rgg);
+    else if (staged)
+        hipLaunchKernelGGL((k_opacity<false, true>), grid, dim3(256), lds, c->stream, c->d, flags,
+                           ffc1, rgg);
     else
-        hipLaunchKernelGGL(k_opacity<false>, grid, dim3(256), 0, c->stream, c->d, flags, ffc1,
-                           (const double *)g_rgg_scratch(c));
+        hipLaunchKernelGGL((k_opacity<false, false>), grid, dim3(256), 0, c->stream, c->d, flags,
+                           ffc1, rgg);
 }
 
 void launch_opacity_finish(slgpu_ctx *c)
