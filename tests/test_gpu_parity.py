@@ -5,6 +5,7 @@ import sys
 
 import numpy as np
 import pytest
+from scipy import stats
 
 from oracle import pyoracle as po
 from superlite_amd import synth
@@ -521,6 +522,32 @@ def coarse(a, nb=20):
     return a[..., :n].reshape(a.shape[:-1] + (nb, -1)).sum(-1)
 
 
+def welch_z(ref_rows, our_rows):
+    """Per bin, the difference of the two samples' per-iteration means over its standard
+    error, as the normal deviate of the same two-sided Welch t p-value (the variances are
+    estimated from as few as 24 rows, so t is used, not the normal).  Bins where both
+    samples are constant are dropped."""
+    a, b = np.asarray(ref_rows, float), np.asarray(our_rows, float)
+    na, nb = len(a), len(b)
+    va, vb = a.var(0, ddof=1) / na, b.var(0, ddof=1) / nb
+    s2 = va + vb
+    ok = s2 > 0
+    va, vb, s2 = va[ok], vb[ok], s2[ok]
+    t = (a.mean(0)[ok] - b.mean(0)[ok]) / np.sqrt(s2)
+    nu = s2 ** 2 / (va ** 2 / (na - 1) + vb ** 2 / (nb - 1))
+    return np.sign(t) * stats.norm.isf(stats.t.sf(np.abs(t), nu))
+
+
+def chi2_bound(dof, p=1e-3):
+    """chi^2/dof above which a sum of dof squared unit normals lies with probability p"""
+    return stats.chi2.isf(p, dof) / dof
+
+
+def zmax_bound(n, p=1e-3):
+    """|z| that the largest of n unit normals exceeds with probability p (Sidak)"""
+    return stats.norm.isf(0.5 * (1.0 - (1.0 - p) ** (1.0 / n)))
+
+
 @pytest.mark.parametrize("case", ["c1mixed", "c1", "c2"])
 def test_transport_statistics_vs_reference(S, tables, golden, case):
     """MC tallies within 3 sigma of SuperLite's own Monte Carlo (golden per-iteration rows)."""
@@ -554,9 +581,24 @@ def test_transport_statistics_vs_reference(S, tables, golden, case):
         return (ma[ok] - mb[ok]) / s[ok]
     for key, f in (("luminos", coarse), ("tally", lambda a: coarse(a, 10))):
         zz = z(f(g[key]), [f(r[key]) for r in rows])
-        say(case, key, "chi2/dof", np.mean(zz ** 2), "max|z|", np.max(np.abs(zz)))
+        say(case, key, "coarse chi2/dof", np.mean(zz ** 2), "max|z|", np.max(np.abs(zz)))
         assert np.mean(zz ** 2) < 2.5
         assert np.max(np.abs(zz)) < 4.5
+    # per group (groups with at least 100 reference escapes over its iterations) and per
+    # shell: chi^2/dof against the chi^2(dof) quantile, the largest |z| against the bound
+    # for that many bins, both at p = 1e-3
+    keep = np.asarray(g["lumnum"]).sum(0) >= 100
+    per = [("tally", np.asarray(g["tally"]), np.array([r["tally"] for r in rows]))]
+    if keep.sum() >= 20:
+        per.append(("luminos", np.asarray(g["luminos"])[:, keep],
+                    np.array([r["luminos"] for r in rows])[:, keep]))
+    for key, ref_rows, our_rows in per:
+        zz = welch_z(ref_rows, our_rows)
+        n = len(zz)
+        say(case, key, "per bin: n", n, "chi2/dof", np.mean(zz ** 2), "bound", chi2_bound(n),
+            "max|z|", np.max(np.abs(zz)), "bound", zmax_bound(n))
+        assert np.mean(zz ** 2) < chi2_bound(n)
+        assert np.max(np.abs(zz)) < zmax_bound(n)
     for key in ("eout", "evelo"):
         zz = z(g[key][:, None], [[r[key]] for r in rows])
         assert abs(zz[0]) < 4.0, (key, zz)
