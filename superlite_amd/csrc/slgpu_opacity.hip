@@ -118,25 +118,6 @@ __global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const doubl
     }
 }
 
-// exp(-x) for x >= 0 in FP32 for the precision form's line terms: 2^(-x log2 e) with the
-// rounding error of the product -x*log2(e) (its fma residual plus the tail of log2 e)
-// applied to first order.  About 1.5 ulp, against ~1 ulp for expf at half the instructions.
-__device__ __forceinline__ float exp_neg32(float x)
-{
-    const float y = -x * 1.44269502f;
-    const float e = fmaf(-x, 1.44269502f, -y) + -x * 1.925963e-8f;
-    const float p = __builtin_amdgcn_exp2f(y);
-    return fmaf(p, e * 0.693147181f, p);
-}
-
-// order LDS writes of a wave before its later reads (each wave owns its slot)
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Fused bb + bf + ff for one (cell, group); result into capT[g*nc + i] (cell-fastest).
 // blockDim = 256 = 64 cells x 4 groups; grid = (ceil(nc/64), ceil(ng/4)).
 // S: the block's 64 cells' line populations oc*ginv of every ion are staged in LDS once

@@ -7,8 +7,10 @@
 //   k_vjp_means     one block per cell: the means' gradients folded into dL/dcap, plus
 //                   their direct T and sigma_Thomson terms
 //   k_vjp_opacity   one thread per (cell, group), 64 cells x 16 groups per block: the bb
-//                   line terms recomputed (not stored), bf and ff, with per-ion and
+//                   line terms recomputed (not stored) and ff, with per-ion and
 //                   per-element accumulators in LDS flushed once per block
+//   k_vjp_bf        the bf part, a contraction over groups of the cross sections with
+//                   dL/dcap
 //   k_vjp_finish    one thread per cell: chain rule through hckt = hc/kT, the ff prefactor
 //                   (natom/V)^2 n_e and sigma_T = c_T n_e natom/V
 //
@@ -17,6 +19,7 @@
 // bb line terms are FP32 with Kahan-compensated sums, like that form's forward; bf, ff,
 // the means and every accumulator stay FP64.  Elements absent from every cell are skipped
 // by the forward's ff sum, so their mass-fraction gradient is returned as 0.
+#include <type_traits>
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
 
@@ -119,19 +122,36 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v)
 
 // dynamic LDS: s_oc[nion][64] then s_fr[nel][64].  M: the precision form, whose bb line
 // terms are recomputed in FP32 with Kahan-compensated sums as in its forward.
-template <bool M>
+// bb and ff of the backward.  LDS: per-ion sums of G*base over the block's lines
+// (s_bb; multiplied by ginv at the flush, since d(oc*ginv*base)/d(oc) = ginv*base and ginv
+// is constant over lines and groups), per-element ff sums (s_fr), and with S the staged
+// line populations of the block's 64 cells (FP32 in the precision form) and each wave's slot
+// of 64 line records.  A population that is not positive contributes nothing (the forward
+// skips it); it is staged as -1 in the precision form, so that a positive one that
+// underflows FP32 (+0) still contributes its derivative, and as 0 in FP64.  The bf part is k_vjp_bf.
+template <bool M, bool S>
 __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ffc1,
                                                      const double *__restrict__ rgg_tab)
 {
+    using PT = typename std::conditional<M, float, double>::type;
     extern __shared__ double s_acc[];
+    __shared__ LineRec s_lines[4][64];
     const int nc = d.nx, ng = d.ng, nion = d.nion, nel = d.nel;
-    double *s_oc = s_acc, *s_fr = s_acc + (size_t)nion * 64;
+    double *s_bb = s_acc, *s_fr = s_acc + (size_t)nion * 64;
+    PT *s_pop = reinterpret_cast<PT *>(s_acc + (size_t)(nion + nel) * 64);
     for (int k = threadIdx.x; k < (nion + nel) * 64; k += 256) s_acc[k] = 0.0;
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     const bool act = i < nc;
     const int ic = act ? i : nc - 1;
+    if constexpr (S) {
+        for (int k = threadIdx.x; k < nion * 64; k += 256) {
+            const int cc = blockIdx.x * 64 + (k & 63);
+            const double o = cc < nc ? d.ocg[(size_t)(k >> 6) * nc + cc] : 0.0;
+            s_pop[k] = o > 0.0 ? (PT)o : (PT)(M ? -1 : 0);
+        }
+    }
+    __syncthreads();
     const double hckt = d.hckt[ic];
     double dh_acc = 0.0, dhl_acc = 0.0;
     for (int q = 0; q < VJP_GROUPS / 4; ++q) {
@@ -152,40 +172,80 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
             if constexpr (M) {
                 const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
                 float a0 = 0.0f, c0 = 0.0f, a1 = 0.0f, c1 = 0.0f;
-                for (int l = l0; l < l1; ++l) {
-                    const LineRec r = d.lines[l];
-                    const double o = ocg[(size_t)r.ion * nc];
-                    if (o > 0.0) {
-                        const float base = phi32 * r.gxs * expf(-(r.chilw * hk32)) * ef32;
-                        const float term = base * (float)o;
-                        kahan_add(a0, c0, term);
-                        kahan_add(a1, c1, term * r.chilw);
-                        atomicAdd(&s_oc[r.ion * 64 + lane], G * (double)base * d.ginvT[(size_t)r.ion * nc + ic]);
+                if constexpr (S) {
+                    LineRec *ln = s_lines[wave];
+                    for (int base0 = l0; base0 < l1; base0 += 64) {
+                        const int n = l1 - base0 < 64 ? l1 - base0 : 64;
+                        wave_lds_sync();
+                        if (lane < n) {
+                            LineRec r = d.lines[base0 + lane];
+                            r.gxs = phi32 * r.gxs;             // base's first product
+                            ln[lane] = r;
+                        }
+                        wave_lds_sync();
+                        #pragma unroll 2
+                        for (int j = 0; j < n; ++j) {
+                            const LineRec r = ln[j];
+                            const float of = s_pop[r.ion * 64 + lane];
+                            if (of >= 0.0f) {
+                                const float base = r.gxs * exp_neg32(r.chilw * hk32) * ef32;
+                                const float term = base * of;
+                                kahan_add(a0, c0, term);
+                                kahan_add(a1, c1, term * r.chilw);
+                                atomicAdd(&s_bb[r.ion * 64 + lane], G * (double)base);
+                            }
+                        }
+                    }
+                } else {
+                    for (int l = l0; l < l1; ++l) {
+                        const LineRec r = d.lines[l];
+                        const double o = ocg[(size_t)r.ion * nc];
+                        if (o > 0.0) {
+                            const float base = phi32 * r.gxs * exp_neg32(r.chilw * hk32) * ef32;
+                            const float term = base * (float)o;
+                            kahan_add(a0, c0, term);
+                            kahan_add(a1, c1, term * r.chilw);
+                            atomicAdd(&s_bb[r.ion * 64 + lane], G * (double)base);
+                        }
                     }
                 }
                 s0 = (double)a0;
                 s1 = (double)a1;
             } else {
-                for (int l = l0; l < l1; ++l) {
-                    const LineRec r = d.lines[l];
-                    const double o = ocg[(size_t)r.ion * nc];
-                    if (o > 0.0) {
-                        const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
-                        const double term = base * o;
-                        s0 += term;
-                        s1 += term * (double)r.chilw;
-                        atomicAdd(&s_oc[r.ion * 64 + lane], G * base * d.ginvT[(size_t)r.ion * nc + ic]);
+                if constexpr (S) {
+                    LineRec *ln = s_lines[wave];
+                    for (int base0 = l0; base0 < l1; base0 += 64) {
+                        const int n = l1 - base0 < 64 ? l1 - base0 : 64;
+                        wave_lds_sync();
+                        if (lane < n) ln[lane] = d.lines[base0 + lane];
+                        wave_lds_sync();
+                        for (int j = 0; j < n; ++j) {
+                            const LineRec r = ln[j];
+                            const double o = s_pop[r.ion * 64 + lane];
+                            if (o > 0.0) {
+                                const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
+                                const double term = base * o;
+                                s0 += term;
+                                s1 += term * (double)r.chilw;
+                                atomicAdd(&s_bb[r.ion * 64 + lane], G * base);
+                            }
+                        }
+                    }
+                } else {
+                    for (int l = l0; l < l1; ++l) {
+                        const LineRec r = d.lines[l];
+                        const double o = ocg[(size_t)r.ion * nc];
+                        if (o > 0.0) {
+                            const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
+                            const double term = base * o;
+                            s0 += term;
+                            s1 += term * (double)r.chilw;
+                            atomicAdd(&s_bb[r.ion * 64 + lane], G * base);
+                        }
                     }
                 }
             }
             dh_acc += G * (-s1 + s0 * wlinv * ec / expfac);
-        }
-        if (!(flags & SLGPU_NOBF)) {
-            const double *xs = d.bfxs + (size_t)(g - 1) * nion;
-            for (int k = 0; k < nion; ++k) {
-                const double x = xs[k];
-                if (x != 0.0) atomicAdd(&s_oc[k * 64 + lane], G * x);
-            }
         }
         if (!(flags & SLGPU_NOFF)) {
             const double wl = wlo;
@@ -230,10 +290,41 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
         const int row = k >> 6, cell = blockIdx.x * 64 + (k & 63);
         const double v = s_acc[k];
         if (cell < nc && v != 0.0) {
-            if (row < nion) atomicAdd(&d.goc[(size_t)row * nc + cell], v);
+            if (row < nion) atomicAdd(&d.goc[(size_t)row * nc + cell], v * d.ginvT[(size_t)row * nc + cell]);
             else atomicAdd(&d.gfr[(size_t)(row - nion) * nc + cell], v);
         }
     }
+}
+
+// bf part of the backward: d(sum_k bfxs(g,k) oc(k,i))/d(oc(k,i)) summed against G(g,i), a
+// contraction over the groups, goc(k,i) += sum_g bfxs(g,k) G(g,i).  One lane per cell,
+// BF_IONS ions and BF_GROUPS groups per block: the lane keeps its ions' partial sums in
+// registers and reads its cell's G once per group (the group's cross sections are
+// wave-uniform loads); the partial sums of the group chunks meet in FP64 atomics.
+constexpr int BF_IONS = 16, BF_GROUPS = 128;
+__global__ __launch_bounds__(64) void k_vjp_bf(Dev d)
+{
+    const int nc = d.nx, ng = d.ng, nion = d.nion;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int k0 = blockIdx.y * BF_IONS;
+    const int g0 = blockIdx.z * BF_GROUPS, g1 = g0 + BF_GROUPS < ng ? g0 + BF_GROUPS : ng;
+    if (i >= nc) return;
+    double acc[BF_IONS];
+    #pragma unroll
+    for (int q = 0; q < BF_IONS; ++q) acc[q] = 0.0;
+    for (int g = g0; g < g1; ++g) {
+        const double G = d.gcapT[(size_t)g * nc + i];
+        const double *xs = d.bfxs + (size_t)g * nion + k0;
+        #pragma unroll
+        for (int q = 0; q < BF_IONS; ++q)
+            if (k0 + q < nion) {
+                const double x = xs[q];
+                if (x != 0.0) acc[q] += G * x;
+            }
+    }
+    #pragma unroll
+    for (int q = 0; q < BF_IONS; ++q)
+        if (k0 + q < nion && acc[q] != 0.0) atomicAdd(&d.goc[(size_t)(k0 + q) * nc + i], acc[q]);
 }
 
 // chain rule to the inputs; outputs in the forward's input layouts
@@ -266,12 +357,25 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     (void)hipMemsetAsync(c->d.goc, 0, (size_t)std::max(c->nion, 1) * nc * 8, c->stream);
     (void)hipMemsetAsync(c->d.gfr, 0, (size_t)30 * nc * 8, c->stream);
     hipLaunchKernelGGL(k_vjp_means, dim3(nc), dim3(256), 0, c->stream, c->d, gc, gp, gr);
-    const size_t lds = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
+    const bool mixed = c->d.precision == SLGPU_MIXED;
+    const size_t acc_b = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
+    const size_t pop_b = (flags & SLGPU_NOBB) ? 0 : (size_t)c->nion * 64 * (mixed ? sizeof(float) : sizeof(double));
+    // staged populations while the block's LDS stays within 64 KiB (4 KiB of line slots)
+    const bool staged = acc_b + pop_b + 4096 <= 65536;
+    const size_t lds = acc_b + (staged ? pop_b : 0);
     dim3 grid((nc + 63) / 64, (ng + VJP_GROUPS - 1) / VJP_GROUPS);
-    if (c->d.precision == SLGPU_MIXED)
-        hipLaunchKernelGGL(k_vjp_opacity<true>, grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    if (mixed && staged)
+        hipLaunchKernelGGL((k_vjp_opacity<true, true>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    else if (mixed)
+        hipLaunchKernelGGL((k_vjp_opacity<true, false>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    else if (staged)
+        hipLaunchKernelGGL((k_vjp_opacity<false, true>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
     else
-        hipLaunchKernelGGL(k_vjp_opacity<false>, grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+        hipLaunchKernelGGL((k_vjp_opacity<false, false>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    if (!(flags & SLGPU_NOBF) && c->nion > 0)
+        hipLaunchKernelGGL(k_vjp_bf,
+                           dim3((nc + 63) / 64, (c->nion + BF_IONS - 1) / BF_IONS, (ng + BF_GROUPS - 1) / BF_GROUPS),
+                           dim3(64), 0, c->stream, c->d);
     hipLaunchKernelGGL(k_vjp_finish, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d,
                        flags, gT, gnat, gne, gfr, goc);
 }
