@@ -76,9 +76,7 @@ def cpu_baseline(w, st, wl, lines, tables, target_s=10.0):
                 note="FP64 oracle restatement on the host cores; a baseline, not a target")
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
-    known size) from the committed rocprofv3 counter summary, if it covers this workload."""
+def _counter_record(kernel, workload):
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             rec = json.load(f).get(kernel)
@@ -86,7 +84,27 @@ def measured_traffic(kernel, workload):
         return None
     if not rec or rec.get("workload") != workload:
         return None
-    return rec["fetch_bytes"] + rec["write_bytes"]
+    return rec
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size) from the committed rocprofv3 counter summary, if it covers this workload."""
+    rec = _counter_record(kernel, workload)
+    return None if rec is None else rec["fetch_bytes"] + rec["write_bytes"]
+
+
+# VALU issue rate of MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def valu_issue_ms(kernel, workload):
+    """Time the committed SQ_INSTS_VALU count of one launch of `kernel` takes at the VALU
+    issue rate, or None."""
+    rec = _counter_record(kernel, workload)
+    if rec is None or "valu_wave_insts" not in rec:
+        return None
+    return rec["valu_wave_insts"] / VALU_ISSUE_PER_S * 1e3
 
 
 def calibration_copy(mib):
@@ -309,7 +327,13 @@ def main():
                                      ("v_exp_f32/s" if mixed else "FP64 exp()/s"),
                              "frac": line_evals / t_opk / exp_peak, "kernel_ms": t_opk * 1e3,
                              "hbm_achieved_gbs": opac_alg_bytes / t_opk / 1e9,
-                             "hbm_frac": opac_alg_bytes / t_opk / 1e9 / PEAK_HBM_GBS},
+                             "hbm_frac": opac_alg_bytes / t_opk / 1e9 / PEAK_HBM_GBS,
+                             # the binding resource: its VALU instructions (rocprofv3
+                             # SQ_INSTS_VALU of one launch) at the issue rate, over the
+                             # measured time
+                             "valu_issue_ms": valu_issue_ms("k_opacity<true>" if mixed else "-", args.workload),
+                             "valu_issue_frac": (lambda v: None if v is None else v / (t_opk * 1e3))(
+                                 valu_issue_ms("k_opacity<true>" if mixed else "-", args.workload))},
         "measured_peaks": {"exp32_per_s": exp32, "exp64_per_s": exp64},
     }
     if slgpu.SLGPU_FP64 in runs and mixed:
