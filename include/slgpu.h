@@ -160,6 +160,38 @@ int slgpu_fluxtally(slgpu_ctx *ctx, double *out_luminos, double *out_lumdev,
 int slgpu_get_particles(slgpu_ctx *ctx, int64_t n, uint8_t *stat, double *x, double *mu,
                         double *wl, double *e, int *ig, int64_t *nstep, int *err);
 
+/* NLTE hydrogen data (nltemod.f:88-166 read_atom_nlte + read_nlte_data.f:148-233): the H I
+ * model atom as read_nlte_data stores it.  glev (nlev) statistical weights; per radiative
+ * line (nlin): llw < lup (1-based levels), wl0 [Angstrom] from the level energies (the file's
+ * value where they give none), f, gl, gu, aul, bul, blu; per electron-impact line (ncoll):
+ * llw_coll < lup_coll, coll_c (4 fit coefficients per line), dele [eV]; per level: lev_pi
+ * (the level of each PI/RR record, every level once), pi_c and rr_c (4 fit coefficients per
+ * record), dele_pi [eV].  Level pairs must be distinct within each kind.  Call after
+ * slgpu_set_groups and slgpu_set_ions (the group of each line and the H I ion index are
+ * taken then).  superlite_amd/io.py reads the data.nlte.* files into these arrays. */
+int slgpu_set_nlte_data(slgpu_ctx *ctx, int nlev, int nlin, int ncoll, const double *glev,
+                        const int *llw, const int *lup, const double *wl0, const double *f,
+                        const double *gl, const double *gu, const double *aul, const double *bul,
+                        const double *blu, const int *llw_coll, const int *lup_coll,
+                        const double *coll_c, const double *dele, const int *lev_pi,
+                        const double *pi_c, const double *rr_c, const double *dele_pi);
+
+/* GAS/physical_opacity_nlte.f:3-708 + opacity_planckmean.f + opacity_rossmean.f with H I in
+ * NLTE (nlte_nelem = 1): per massive cell the H I level populations from the rate equations
+ * (radiative rates with J = B_nu(T) at the group centre, electron-impact rates, PI/RR with
+ * radtemp; solved by LU with partial pivoting), the H lines' NLTE absorption and emission,
+ * the other ions' lines in LTE (H I lines of the context's line list are skipped), bf and ff.
+ * radtemp (nc) = grd_radtemp; ndens (2*nc) = nlte_ndens(i,1:2,1), the H I and H II number
+ * densities of eos_update.f:52-57.  Outputs as slgpu_opacity_build, plus out_capemit
+ * (gas_capemit, ng*nc FP32), out_capemitgrey (nc) and out_npop (nlev*nc, level-fastest, the
+ * normalised populations); any may be NULL.  FP64 arithmetic in both precision forms.
+ * SLGPU_EINVAL where the reference stops on a singular rate matrix. */
+int slgpu_opacity_build_nlte(slgpu_ctx *ctx, const slgpu_gas *gas, const double *radtemp,
+                             const double *ndens, int flags, double *out_cap64, float *out_cap32,
+                             float *out_capemit, double *out_sig, double *out_capgrey,
+                             double *out_capemitgrey, double *out_capross, float *out_emiss,
+                             double *out_npop);
+
 /* Differentiation form: vector-Jacobian product of the last slgpu_opacity_build.
  * Backward of physical_opacity.f:93-247 (bb, bf, ff), opacity_planckmean.f:18-24 and
  * opacity_rossmean.f:18-21 with respect to the build's own inputs.  grad_cap64 (ng*nc, the

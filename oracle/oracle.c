@@ -361,6 +361,332 @@ int or_opacity(const or_opac_in *in, double *cap64, float *cap32, double *sig,
 }
 
 /* ------------------------------------------------------------------------ */
+/* LAPACK dgetf2 (unblocked LU with partial pivoting, row interchanges) and dgetrs (no
+ * transpose) on a column-major n x n matrix; the updates are written as fma so that the
+ * device restatement can reproduce them operation for operation. */
+#define A_(i, j) a[(size_t)(j) * n + (i)]
+int or_lu_solve(int n, double *a, int *ipiv, double *b)
+{
+    int info = 0;
+    const double sfmin = DBL_MIN;
+    for (int j = 0; j < n; ++j) {
+        int jp = j;
+        double amax = fabs(A_(j, j));
+        for (int i = j + 1; i < n; ++i)                 /* idamax: first largest |a| */
+            if (fabs(A_(i, j)) > amax) { amax = fabs(A_(i, j)); jp = i; }
+        ipiv[j] = jp;
+        if (A_(jp, j) != 0.0) {
+            if (jp != j)
+                for (int k = 0; k < n; ++k) { double t = A_(j, k); A_(j, k) = A_(jp, k); A_(jp, k) = t; }
+            if (fabs(A_(j, j)) >= sfmin) {
+                const double r = 1.0 / A_(j, j);
+                for (int i = j + 1; i < n; ++i) A_(i, j) = A_(i, j) * r;
+            } else {
+                for (int i = j + 1; i < n; ++i) A_(i, j) = A_(i, j) / A_(j, j);
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int k = j + 1; k < n; ++k) {               /* dger: a -= l u */
+            const double u = A_(j, k);
+            if (u != 0.0)
+                for (int i = j + 1; i < n; ++i) A_(i, k) = fma(-A_(i, j), u, A_(i, k));
+        }
+    }
+    if (info) return info;
+    for (int k = 0; k < n; ++k)                          /* dlaswp */
+        if (ipiv[k] != k) { double t = b[k]; b[k] = b[ipiv[k]]; b[ipiv[k]] = t; }
+    for (int k = 0; k < n; ++k)                          /* dtrsm lower, unit */
+        if (b[k] != 0.0)
+            for (int i = k + 1; i < n; ++i) b[i] = fma(-b[k], A_(i, k), b[i]);
+    for (int k = n - 1; k >= 0; --k)                     /* dtrsm upper, non-unit */
+        if (b[k] != 0.0) {
+            b[k] = b[k] / A_(k, k);
+            for (int i = 0; i < k; ++i) b[i] = fma(-b[k], A_(i, k), b[i]);
+        }
+    return 0;
+}
+#undef A_
+
+/* physical_opacity_nlte.f:148-263: the H I level populations of cell i (normalised to 1) */
+static int or_nlte_pops(const or_opac_in *in, const or_nlte_in *nl, int i, double *rc,
+                        double *a, int *ipiv, double *gpi, double *rhs)
+{
+    const int n = nl->nlev, ng = in->ng;
+    const double *wlsub = in->wl;
+    const double kb_ev = PC_KB / PC_EV;
+    const double tempinv = 1.0 / in->temp[i];
+#define RC(l, ll) rc[(size_t)((ll) - 1) * n + ((l) - 1)]
+    for (size_t k = 0; k < (size_t)n * n; ++k) rc[k] = 0.0;
+    for (int l = 0; l < n; ++l) rhs[l] = 0.0;
+    rhs[0] = 1.0;
+    /* radiative terms, :160-184 (J at the group centre, Planck at the gas temperature) */
+    for (int il = 0; il < nl->nlin; ++il) {
+        const double wl0 = nl->wl0[il] * PC_ANG;
+        int igs;
+        for (igs = 0; igs <= ng; ++igs)
+            if (wlsub[igs] > wl0) break;                 /* wlsub(igs+1) > wl0 */
+        if (igs < 1 || igs > ng) continue;
+        const double wlm = 0.5 * (wlsub[igs] + wlsub[igs - 1]);
+        const double jnu = (wlm * wlm / PC_C) * or_planck(wlm, in->temp[i]);
+        RC(nl->llw[il], nl->lup[il]) = nl->blu[il] * jnu;
+        RC(nl->lup[il], nl->llw[il]) = nl->aul[il] + nl->bul[il] * jnu;
+    }
+    /* electron-impact excitation, :189-222 */
+    for (int il = 0; il < nl->ncoll; ++il) {
+        const double x = nl->dele[il] * tempinv / kb_ev;
+        const double lx = log(x);
+        const double *c = nl->coll_c + 4 * (size_t)il;
+        double f = c[0] + c[1] * lx + c[2] * (lx * lx) + c[3] * (lx * lx * lx);
+        f = exp(f);
+        const double gamma = 1.58e-5 * (f / x) * exp(-1.0 * x) * pow(tempinv, 1.5) / pow(kb_ev, 1.5);
+        const int lw = nl->llw_coll[il], lu = nl->lup_coll[il];
+        const double gamma_d = nl->glev[lw - 1] * gamma * exp(x) / nl->glev[lu - 1];
+        const double clu = gamma * in->nelec[i] * in->natom[i] / in->vol[i];
+        const double cul = gamma_d * in->nelec[i] * in->natom[i] / in->vol[i];
+        RC(lw, lu) = RC(lw, lu) + clu;
+        RC(lu, lw) = RC(lu, lw) + cul;
+    }
+    /* photoionisation and radiative recombination, :226-253 */
+    for (int ll = 0; ll < n; ++ll) {
+        const int l = nl->lev_pi[ll];
+        const double xpi = nl->dele_pi[ll] * tempinv / kb_ev;
+        const double lp = log(xpi);
+        const double *cp = nl->pi_c + 4 * (size_t)ll, *cr = nl->rr_c + 4 * (size_t)ll;
+        double fpi = cp[0] + cp[1] * lp + cp[2] * (lp * lp) + cp[3] * (lp * lp * lp);
+        fpi = exp(fpi);
+        gpi[l - 1] = (fpi / xpi) * exp(-1.0 * xpi) * pow(tempinv, 1.5) / pow(kb_ev, 1.5);
+        const double xrr = kb_ev * nl->radtemp[i];
+        const double lr = log(xrr);
+        double frr = cr[0] + cr[1] * lr + cr[2] * (lr * lr) + cr[3] * (lr * lr * lr);
+        frr = exp(frr);
+        if (l > 1) rhs[l - 1] = -1.0 * frr * nl->ndens[2 * (size_t)i + 1] / nl->ndens[2 * (size_t)i];
+    }
+    /* rate matrix, :255-266 (row 1: the populations sum to 1) */
+    for (int l = 1; l <= n; ++l)
+        for (int ll = 1; ll <= n; ++ll) {
+            double v;
+            if (l == 1) v = 1.0;
+            else if (l == ll) {
+                double sum = 0.0;
+                for (int k = 1; k <= n; ++k) sum += RC(l, k);
+                v = -1.0 * sum;
+                v = v - gpi[l - 1];
+            } else v = RC(ll, l);
+            a[(size_t)(ll - 1) * n + (l - 1)] = v;
+        }
+#undef RC
+    return ipiv ? or_lu_solve(n, a, ipiv, rhs) : 0;
+}
+
+int or_nlte_rate_matrix(const or_opac_in *in, const or_nlte_in *nl, int i, double *a, double *b)
+{
+    const int n = nl->nlev;
+    double *rc = malloc(sizeof(double) * n * n), *gpi = calloc(n, sizeof(double));
+    or_nlte_pops(in, nl, i, rc, a, NULL, gpi, b);
+    free(rc); free(gpi);
+    return 0;
+}
+
+int or_opacity_nlte(const or_opac_in *in, const or_nlte_in *nl, double *cap64, float *cap32,
+                    float *capemit32, double *sig, double *capgrey, double *capemitgrey,
+                    double *capross, float *emiss, double *npop)
+{
+    const int nc = in->nc, ng = in->ng, n = nl->nlev;
+    const double *wlsub = in->wl;
+    const double cthomson = 8.0 * PC_PI * (PC_E * PC_E * PC_E * PC_E) /
+                            (3.0 * (PC_ME * PC_ME) * (PC_C * PC_C * PC_C * PC_C));
+    const double c1 = 4.0 * (PC_E * PC_E * PC_E * PC_E * PC_E * PC_E) /
+                      (3.0 * PC_H * PC_ME * (PC_C * PC_C * PC_C * PC_C)) *
+                      sqrt(PC_PI2 / (3 * PC_ME * PC_H * PC_C));
+    const double fconst = PC_PI * (PC_E * PC_E) / (PC_ME * PC_C);
+    double *hckt = malloc(sizeof(double) * nc);
+    double *expfac = malloc(sizeof(double) * nc);
+    double *cap = calloc((size_t)nc * ng, sizeof(double));
+    double *capemit = calloc((size_t)nc * ng, sizeof(double));
+    double *em = calloc((size_t)nc * ng, sizeof(double));
+    double *pops = calloc((size_t)nc * n, sizeof(double));
+    double *rc = malloc(sizeof(double) * n * n), *a = malloc(sizeof(double) * n * n);
+    double *gpi = calloc(n, sizeof(double)), *rhs = malloc(sizeof(double) * n);
+    int *ipiv = malloc(sizeof(int) * n);
+    int rc_ = 0;
+    int lut[31][32];
+    for (int x = 0; x < 31; ++x)
+        for (int y = 0; y < 32; ++y) lut[x][y] = -1;
+    for (int k = 0; k < in->nion; ++k) lut[in->ion_iz[k]][in->ion_ii[k]] = k;
+    for (int i = 0; i < nc; ++i) hckt[i] = PC_H * PC_C / (PC_KB * in->temp[i]);
+    for (int i = 0; i < nc; ++i) {
+        if (in->flags & 8) sig[i] = 0.0;
+        else sig[i] = cthomson * in->nelec[i] * in->natom[i] / in->vol[i];
+        if (in->mass[i] <= 0.0) sig[i] = 0.0;
+    }
+#define AT(arr, i, igs) arr[(i) + (size_t)nc * ((igs) - 1)]
+    if (!(in->flags & 1)) {
+        /* populations, :140-263 */
+        for (int i = 0; i < nc && !rc_; ++i) {
+            if (in->mass[i] <= 0.0) continue;
+            if (or_nlte_pops(in, nl, i, rc, a, ipiv, gpi, rhs)) rc_ = 1;
+            for (int l = 0; l < n; ++l) pops[(size_t)i * n + l] = rhs[l];
+        }
+        /* NLTE H lines, :304-345 */
+        int dirty = 1;
+        double phi = 0.0, wl = 0.0;
+        for (int il = 0; il < nl->nlin; ++il) {
+            const double wl0 = nl->wl0[il] * PC_ANG;
+            int igs;
+            for (igs = 0; igs <= ng; ++igs) {
+                if (wlsub[igs] > wl0) break;
+                dirty = 1;
+            }
+            if (igs < 1 || igs > ng) continue;
+            if (dirty) {
+                dirty = 0;
+                wl = 0.5 * (wlsub[igs - 1] + wlsub[igs]);
+                const double dwl = wlsub[igs] - wlsub[igs - 1];
+                phi = wl * wl / (dwl * PC_C);
+            }
+            const int lw = nl->llw[il], lu = nl->lup[il];
+            const double g_llw = nl->gl[il], g_lup = nl->gu[il];
+            for (int i = 0; i < nc; ++i) {
+                if (in->mass[i] <= 0.0) continue;
+                const double npl = pops[(size_t)i * n + lw - 1] * nl->ndens[2 * (size_t)i];
+                const double npu = pops[(size_t)i * n + lu - 1] * nl->ndens[2 * (size_t)i];
+                const double caphelp = npl * fconst * nl->f[il] * phi * (1.0 - (npu * g_llw / (npl * g_lup)));
+                AT(cap, i, igs) += caphelp;
+                const double bnu = (wl * wl / PC_C) * or_planck(wl, in->temp[i]);
+                const double emithelp = npu * nl->aul[il] * phi * PC_H * PC_C / (4.0 * PC_PI * wl0);
+                AT(capemit, i, igs) += emithelp / bnu;
+                AT(em, i, igs) += emithelp;
+            }
+        }
+        /* LTE lines of the other elements, :349-401 */
+        int iigs = 0;
+        dirty = 1;
+        phi = 0.0;
+        for (int il = 0; il < in->nline; ++il) {
+            const double wl0 = (double)in->wl0[il] * PC_ANG;
+            int igs;
+            for (igs = iigs; igs <= ng; ++igs) {
+                if (wlsub[igs] > wl0) break;
+                dirty = 1;
+            }
+            iigs = igs;
+            if (igs < 1 || igs > ng) continue;
+            if (dirty) {
+                dirty = 0;
+                wl = 0.5 * (wlsub[igs - 1] + wlsub[igs]);
+                const double wlinv = 1.0 / wl;
+                const double dwl = wlsub[igs] - wlsub[igs - 1];
+                for (int i = 0; i < nc; ++i) expfac[i] = 1.0 - exp(-hckt[i] * wlinv);
+                phi = wl * wl / (dwl * PC_C);
+            }
+            const int k = lut[in->liz[il]][in->lii[il]];
+            if (k < 0) continue;
+            const double gxs = (double)in->gxs[il], chilw = (double)in->chilw[il];
+            for (int i = 0; i < nc; ++i) {
+                if (in->mass[i] <= 0.0) continue;
+                const double ocg = in->oc[k + (size_t)in->nion * i] * in->ginv[k + (size_t)in->nion * i];
+                if (ocg <= 0.0) continue;
+                const double caphelp = phi * gxs * ocg * exp(-chilw * hckt[i]) * expfac[i];
+                AT(cap, i, igs) += caphelp;
+                AT(capemit, i, igs) += caphelp;
+                const double bnu = (wl * wl / PC_C) * or_planck(wl, in->temp[i]);
+                AT(em, i, igs) += caphelp * bnu;
+            }
+        }
+    }
+    /* bound-free, :413-452 (left group edge) */
+    if (!(in->flags & 2)) {
+        for (int igs = 1; igs <= ng; ++igs) {
+            const double wl = wlsub[igs - 1];
+            const double en = PC_H * PC_C / (PC_EV * wl);
+            for (int k = 0; k < in->nion; ++k) {
+                const int iz = in->ion_iz[k], ii = in->ion_ii[k];
+                const double xs = or_bfxs(iz, iz - ii + 1, en, in->bf_ph1, in->bf_ph2);
+                if (xs == 0.0) continue;
+                for (int i = 0; i < nc; ++i) {
+                    if (in->mass[i] <= 0.0) continue;
+                    const double o = in->oc[k + (size_t)in->nion * i];
+                    AT(cap, i, igs) += xs * PC_MBARN * o;
+                    AT(capemit, i, igs) += xs * PC_MBARN * o;
+                    const double bnu = (wl * wl / PC_C) * or_planck(wl, in->temp[i]);
+                    AT(em, i, igs) += xs * PC_MBARN * o * bnu;
+                }
+            }
+        }
+    }
+    /* free-free, :458-511 (left group edge) */
+    if (!(in->flags & 4)) {
+        for (int igs = 1; igs <= ng; ++igs) {
+            const double wl = wlsub[igs - 1];
+            const double wlinv = 1.0 / wl;
+            for (int i = 0; i < nc; ++i) {
+                if (in->mass[i] <= 0.0) continue;
+                const double nv = in->natom[i] / in->vol[i];
+                const double hlp = nv * nv * in->nelec[i];
+                const double u = hckt[i] * wlinv;
+                const double ru = 10.0 * (log10(u) + 4.0) + 1.0;
+                int iu = (int)floor(ru);
+                if (iu < 1) iu = 1;
+                if (iu > FF_NU - 1) iu = FF_NU - 1;
+                const double du = ru - iu;
+                const double help = c1 * sqrt(hckt[i]) * (1.0 - exp(-u)) * (wl * wl * wl) * hlp;
+                for (int iz = 1; iz <= 30; ++iz) {
+                    const double gg = (double)(iz * iz) * PC_RYDBERG * hckt[i];
+                    const double rgg = 5.0 * (log10(gg) + 4.0) + 1.0;
+                    int igg = (int)floor(rgg);
+                    if (igg < 1) igg = 1;
+                    if (igg > FF_NGG - 1) igg = FF_NGG - 1;
+                    const double dgg = rgg - igg;
+                    const double *T = in->ff_gff;
+#define GFF(x, y) T[((y) - 1) * FF_NU + ((x) - 1)]
+                    double gff = (1.0 - du) * (1.0 - dgg) * GFF(iu, igg) + du * (1.0 - dgg) * GFF(iu + 1, igg) +
+                                 (1.0 - du) * dgg * GFF(iu, igg + 1) + du * dgg * GFF(iu + 1, igg + 1);
+#undef GFF
+                    if (rgg > FF_NGG) gff = gff > 1.0 ? gff : 1.0;
+                    const double t = help * gff * (double)(iz * iz) * in->natomfr[(iz - 1) + 30 * (size_t)i];
+                    AT(cap, i, igs) += t;
+                    AT(capemit, i, igs) += t;
+                    const double bnu = (wl * wl / PC_C) * or_planck(wl, in->temp[i]);
+                    AT(em, i, igs) += t * bnu;
+                }
+            }
+        }
+    }
+    /* save (:554-556) and the means */
+    for (int i = 0; i < nc; ++i)
+        for (int ig = 1; ig <= ng; ++ig) {
+            const size_t o = (ig - 1) + (size_t)ng * i;
+            if (cap64) cap64[o] = AT(cap, i, ig);
+            cap32[o] = (float)AT(cap, i, ig);
+            capemit32[o] = (float)AT(capemit, i, ig);
+            if (emiss) emiss[o] = (float)AT(em, i, ig);
+        }
+#undef AT
+    double *wlinv = malloc(sizeof(double) * (ng + 1));
+    double *ss = malloc(sizeof(double) * ng);
+    for (int g = 0; g <= ng; ++g) wlinv[g] = 1.0 / in->wl[g];
+    for (int i = 0; i < nc; ++i) {
+        or_specintv(1.0 / in->temp[i], wlinv, ng, 1, ss);
+        double s1 = 0.0, s2 = 0.0;
+        for (int g = 0; g < ng; ++g) {
+            s1 += (double)cap32[g + (size_t)ng * i] * ss[g];
+            s2 += (double)capemit32[g + (size_t)ng * i] * ss[g];
+        }
+        capgrey[i] = s1;
+        capemitgrey[i] = s2;
+        or_spectintv(1.0 / in->temp[i], wlinv, ng, ss);
+        double s = 0.0;
+        for (int g = 0; g < ng; ++g) s += (1.0 / (sig[i] + (double)cap32[g + (size_t)ng * i])) * ss[g];
+        capross[i] = 1.0 / s;
+    }
+    if (npop) memcpy(npop, pops, sizeof(double) * (size_t)nc * n);
+    free(wlinv); free(ss); free(hckt); free(expfac); free(cap); free(capemit); free(em);
+    free(pops); free(rc); free(a); free(gpi); free(rhs); free(ipiv);
+    return rc_;
+}
+
+/* ------------------------------------------------------------------------ */
 #define CAP(ig, ic) ((double)m->cap[((ic) - 1) * (size_t)m->ng + ((ig) - 1)])
 #define XA(i) (m->xarr[(i) - 1])
 #define VXA(i) (m->vxarr[(i) - 1])

@@ -117,6 +117,48 @@ int or_eos(int nc, const or_eos_data *ed, const double *temp, const double *mass
            double *natomfr, int nion, const int *ion_iz, const int *ion_ii, double *oc,
            double *ginv, int *niter);
 
+/* ---- NLTE hydrogen: GAS/physical_opacity_nlte.f:3-708 ---- */
+typedef struct {
+    /* one NLTE species (H I), the permanent arrays of read_nlte_data.f:148-233; levels and
+     * lines 1-based in the index arrays */
+    int nlev, nlin, ncoll;
+    const double *glev;                       /* nlev */
+    const int *llw, *lup;                     /* nlin, lower level first */
+    const double *wl0, *f, *gl, *gu;          /* nlin; wl0 [Angstrom] from the level energies */
+    const double *aul, *bul, *blu;            /* nlin Einstein coefficients */
+    const int *llw_coll, *lup_coll;           /* ncoll */
+    const double *coll_c;                     /* 4*ncoll fit coefficients C0..C3 */
+    const double *dele;                       /* ncoll transition energy [eV] */
+    const int *lev_pi;                        /* nlev */
+    const double *pi_c, *rr_c;                /* 4*nlev fits (PI, RR) */
+    const double *dele_pi;                    /* nlev [eV] */
+    /* per cell */
+    const double *radtemp;                    /* nc */
+    const double *ndens;                      /* 2*nc: number densities of H I and H II
+                                                 (nlte_ndens(i,1:2,1), eos_update.f:52-57) */
+} or_nlte_in;
+
+/* physical_opacity_nlte for nlte_nelem = 1: per massive cell the rate matrix of the H I
+ * levels (radiative rates with J at the group centre, electron-impact rates, photoionisation
+ * and radiative recombination) solved for the level populations (LU with partial pivoting,
+ * LAPACK dgetf2's unblocked algorithm, and dgetrs's substitutions; the reference inverts the
+ * same matrix with dgetrf/dgetri and multiplies), the H lines' NLTE absorption and emission,
+ * the other elements' lines in LTE, bf and ff, each added to cap, capemit and emiss as the
+ * reference does; then the Planck means of cap and capemit (opacity_planckmean.f:18-24) and
+ * the Rosseland mean.  The line list must hold no H I line (the reference's LTE list
+ * excludes NLTE species).  npop (nlev*nc, level-fastest, may be NULL) receives the
+ * normalised populations.  Returns 0, or 1 when a pivot is zero (the reference stops). */
+int or_opacity_nlte(const or_opac_in *in, const or_nlte_in *nl, double *cap64, float *cap32,
+                    float *capemit32, double *sig, double *capgrey, double *capemitgrey,
+                    double *capross, float *emiss, double *npop);
+
+/* the rate matrix (column-major n x n) and right-hand side of cell i, before the solve */
+int or_nlte_rate_matrix(const or_opac_in *in, const or_nlte_in *nl, int i, double *a, double *b);
+
+/* LU factorisation with partial pivoting (dgetf2) of the column-major n x n matrix a, and the
+ * solution of a x = b in place (dgetrs, no transpose).  Returns 0 or the first zero pivot. */
+int or_lu_solve(int n, double *a, int *ipiv, double *b);
+
 /* ---- DDMC prep: leakage_opacity11.f90:30-171, emission_probability.f90:39-62 ---- */
 void or_leakage(const or_model *m, double *opaclump);
 void or_emission_probability(const or_model *m, double *emitprob);

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libslgpu.so")
 SOURCES = ["slgpu_api.hip", "slgpu_opacity.hip", "slgpu_ddmc.hip", "slgpu_transport.hip",
-           "slgpu_vjp.hip", "slgpu_probe.hip", "slgpu_eos.hip"]
+           "slgpu_vjp.hip", "slgpu_probe.hip", "slgpu_eos.hip", "slgpu_nlte.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",

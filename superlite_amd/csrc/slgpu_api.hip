@@ -520,9 +520,17 @@ namespace {
 // The opacity build (O2-O8) with the per-call staging of the ion populations oc and ginv
 // (2 * nion * nc doubles) at `stage`: a fresh allocation for slgpu_opacity_build, the
 // library's static scratch array for slgpu_opacity_build_noalloc.
+// the NLTE part of a build (physical_opacity_nlte): per-call inputs and extra outputs
+struct NlteCall {
+    const double *radtemp, *ndens;
+    float *out_capemit;
+    double *out_capemitgrey, *out_npop;
+};
+
 int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *out_cap64,
                        float *out_cap32, double *out_sig, double *out_capgrey,
-                       double *out_capross, float *out_emiss, double *stage)
+                       double *out_capross, float *out_emiss, double *stage,
+                       const NlteCall *nlc = nullptr)
 {
     const int nc = c->nx, ng = c->ng, nion = c->nion;
     // present elements (ff adds exactly zero for the others, physical_opacity.f:240-241)
@@ -548,15 +556,34 @@ int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *ou
         HIPCHK(hipMemcpyAsync(doc, gas->oc, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(dgi, gas->ginv, (size_t)nion * nc * 8, hipMemcpyHostToDevice, c->stream));
     }
+    if (nlc) {
+        HIPCHK(hipMemcpyAsync(c->nl.radtemp, nlc->radtemp, nc * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->nl.ndens, nlc->ndens, (size_t)2 * nc * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemsetAsync(c->nl.flag, 0, sizeof(int), c->stream));
+    }
     hipEventRecord(c->ev[22], c->stream);
     launch_cell_prep(c, dn, dv, de, doc, dgi, flags);
     hipEventRecord(c->ev[10], c->stream);
-    launch_opacity(c, flags);
+    if (nlc) launch_opacity_nlte(c, flags, ff_prefactor(), rgg_scratch(c));
+    else launch_opacity(c, flags);
     hipEventRecord(c->ev[11], c->stream);
-    launch_opacity_finish(c);
+    if (nlc) {
+        launch_cell_tables(c);
+        launch_nlte_means(c);
+    } else {
+        launch_opacity_finish(c);
+    }
     hipEventRecord(c->ev[23], c->stream);
     HIPCHK(hipGetLastError());
     const size_t cg = (size_t)nc * ng;
+    if (nlc) {
+        if (nlc->out_capemit)
+            HIPCHK(hipMemcpyAsync(nlc->out_capemit, c->nl.capemit32, cg * 4, hipMemcpyDeviceToHost, c->stream));
+        if (nlc->out_capemitgrey)
+            HIPCHK(hipMemcpyAsync(nlc->out_capemitgrey, c->nl.capemitgrey, nc * 8, hipMemcpyDeviceToHost, c->stream));
+        if (nlc->out_npop)
+            HIPCHK(hipMemcpyAsync(nlc->out_npop, c->nl.pop, (size_t)nc * c->nl.n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
     if (out_cap64) HIPCHK(hipMemcpyAsync(out_cap64, c->d.cap64, cg * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_cap32) HIPCHK(hipMemcpyAsync(out_cap32, c->d.cap32, cg * 4, hipMemcpyDeviceToHost, c->stream));
     if (out_emiss) HIPCHK(hipMemcpyAsync(out_emiss, c->d.emiss, cg * 4, hipMemcpyDeviceToHost, c->stream));
@@ -564,6 +591,11 @@ int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *ou
     if (out_capgrey) HIPCHK(hipMemcpyAsync(out_capgrey, c->d.capgrey, nc * 8, hipMemcpyDeviceToHost, c->stream));
     if (out_capross) HIPCHK(hipMemcpyAsync(out_capross, c->d.capross, nc * 8, hipMemcpyDeviceToHost, c->stream));
     int rc = tm.stop();
+    if (!rc && nlc) {
+        int flag = 0;
+        HIPCHK(hipMemcpy(&flag, c->nl.flag, sizeof(int), hipMemcpyDeviceToHost));
+        if (flag) return fail(SLGPU_EINVAL, "nlte opacity: rate matrix numerically singular");
+    }
     float kms = 0.f;
     hipEventElapsedTime(&kms, c->ev[10], c->ev[11]);
     c->t_ms[5] += kms;
@@ -622,6 +654,141 @@ int slgpu_opacity_build_noalloc(slgpu_ctx *c, const slgpu_gas *gas, int flags,
     if (!stage) return fail(SLGPU_EORDER, "scratch array address not resolved at slgpu_create");
     return opacity_build_impl(c, gas, flags, out_cap64, out_cap32, out_sig, out_capgrey,
                               out_capross, out_emiss, stage);
+}
+
+int slgpu_set_nlte_data(slgpu_ctx *c, int nlev, int nlin, int ncoll, const double *glev,
+                        const int *llw, const int *lup, const double *wl0, const double *f,
+                        const double *gl, const double *gu, const double *aul, const double *bul,
+                        const double *blu, const int *llw_coll, const int *lup_coll,
+                        const double *coll_c, const double *dele, const int *lev_pi,
+                        const double *pi_c, const double *rr_c, const double *dele_pi)
+{
+    if (!c || !glev || !llw || !lup || !wl0 || !f || !gl || !gu || !aul || !bul || !blu ||
+        (ncoll > 0 && (!llw_coll || !lup_coll || !coll_c || !dele)) || !lev_pi || !pi_c ||
+        !rr_c || !dele_pi)
+        return fail(SLGPU_EINVAL, "null argument");
+    if (!(c->have_groups && c->have_ions)) return fail(SLGPU_EORDER, "groups and ions must be set first");
+    if (nlev < 2 || nlev > 1024 || nlin < 1 || ncoll < 0)
+        return fail(SLGPU_EINVAL, "nlte data: need 2 <= nlev <= 1024, nlin >= 1, ncoll >= 0");
+    // distinct level pairs (the reference's rate coefficients are set per pair, so a repeated
+    // radiative pair would keep only its last line, a repeated collision pair would add)
+    std::vector<char> seen((size_t)nlev * nlev, 0), seenc((size_t)nlev * nlev, 0), seenl(nlev, 0);
+    for (int k = 0; k < nlin; ++k) {
+        if (llw[k] < 1 || lup[k] > nlev || !(llw[k] < lup[k]))
+            return fail(SLGPU_EINVAL, "nlte data: radiative lines need 1 <= llw < lup <= nlev");
+        char &v = seen[(size_t)(llw[k] - 1) * nlev + lup[k] - 1];
+        if (v) return fail(SLGPU_EINVAL, "nlte data: repeated radiative level pair");
+        v = 1;
+    }
+    for (int k = 0; k < ncoll; ++k) {
+        if (llw_coll[k] < 1 || lup_coll[k] > nlev || !(llw_coll[k] < lup_coll[k]))
+            return fail(SLGPU_EINVAL, "nlte data: collision lines need 1 <= llw < lup <= nlev");
+        char &v = seenc[(size_t)(llw_coll[k] - 1) * nlev + lup_coll[k] - 1];
+        if (v) return fail(SLGPU_EINVAL, "nlte data: repeated collision level pair");
+        v = 1;
+    }
+    for (int k = 0; k < nlev; ++k) {
+        if (lev_pi[k] < 1 || lev_pi[k] > nlev || seenl[lev_pi[k] - 1])
+            return fail(SLGPU_EINVAL, "nlte data: lev_pi must list every level once");
+        seenl[lev_pi[k] - 1] = 1;
+    }
+    const int ng = c->ng, nc = c->nx;
+    // group of each line: the first grp_wl edge above wl0 (physical_opacity_nlte.f:166-172),
+    // 0 outside the grid; and the lines of each group in file order
+    std::vector<int> igs(nlin), cnt(ng + 2, 0);
+    for (int k = 0; k < nlin; ++k) {
+        const double w = wl0[k] * PC_ANG;
+        int g = 0;
+        while (g <= ng && !(c->h_wl[g] > w)) ++g;
+        igs[k] = (g >= 1 && g <= ng) ? g : 0;
+        if (igs[k]) cnt[igs[k]]++;
+    }
+    std::vector<int> hstart(ng + 2, 0), hidx;
+    for (int g = 1; g <= ng; ++g) hstart[g + 1] = hstart[g] + cnt[g];
+    std::vector<int> fill(hstart);
+    hidx.assign(std::max(hstart[ng + 1], 1), 0);
+    for (int k = 0; k < nlin; ++k)
+        if (igs[k]) hidx[fill[igs[k]]++] = k;
+    int kH = -1;
+    for (int k = 0; k < c->nion; ++k)
+        if (c->h_ion_iz[k] == 1 && c->h_ion_ii[k] == 1) kH = k;
+    NlteDev &q = c->nl;
+    if (c->have_nlte) {
+        void *ps[] = {q.glev, q.wl0, q.f, q.gl, q.gu, q.aul, q.bul, q.blu, q.cc, q.dele, q.pic, q.rrc,
+                      q.delepi, q.llw, q.lup, q.igs, q.cl, q.cu, q.levpi, q.hstart, q.hidx, q.radtemp,
+                      q.ndens, q.rc, q.A, q.pop, q.gpi, q.ipiv, q.flag, q.capemitT, q.emissT,
+                      q.capemit32, q.capemitgrey};
+        for (void *p : ps) dfree(c, p);
+        c->have_nlte = false;
+    }
+    q = NlteDev();
+    q.n = nlev; q.nlin = nlin; q.ncoll = ncoll; q.kH = kH;
+    const int nc1 = std::max(ncoll, 1);
+    int rc = 0;
+    rc |= dalloc(c, &q.glev, nlev);
+    rc |= dalloc(c, &q.wl0, nlin); rc |= dalloc(c, &q.f, nlin); rc |= dalloc(c, &q.gl, nlin);
+    rc |= dalloc(c, &q.gu, nlin); rc |= dalloc(c, &q.aul, nlin); rc |= dalloc(c, &q.bul, nlin);
+    rc |= dalloc(c, &q.blu, nlin);
+    rc |= dalloc(c, &q.cc, (size_t)4 * nc1); rc |= dalloc(c, &q.dele, nc1);
+    rc |= dalloc(c, &q.pic, (size_t)4 * nlev); rc |= dalloc(c, &q.rrc, (size_t)4 * nlev);
+    rc |= dalloc(c, &q.delepi, nlev);
+    rc |= dalloc(c, &q.llw, nlin); rc |= dalloc(c, &q.lup, nlin); rc |= dalloc(c, &q.igs, nlin);
+    rc |= dalloc(c, &q.cl, nc1); rc |= dalloc(c, &q.cu, nc1); rc |= dalloc(c, &q.levpi, nlev);
+    rc |= dalloc(c, &q.hstart, ng + 2); rc |= dalloc(c, &q.hidx, hidx.size());
+    rc |= dalloc(c, &q.radtemp, nc); rc |= dalloc(c, &q.ndens, (size_t)2 * nc);
+    rc |= dalloc(c, &q.rc, (size_t)nc * nlev * nlev); rc |= dalloc(c, &q.A, (size_t)nc * nlev * nlev);
+    rc |= dalloc(c, &q.pop, (size_t)nc * nlev); rc |= dalloc(c, &q.gpi, (size_t)nc * nlev);
+    rc |= dalloc(c, &q.ipiv, (size_t)nc * nlev); rc |= dalloc(c, &q.flag, 1);
+    rc |= dalloc(c, &q.capemitT, (size_t)nc * ng); rc |= dalloc(c, &q.emissT, (size_t)nc * ng);
+    rc |= dalloc(c, &q.capemit32, (size_t)nc * ng); rc |= dalloc(c, &q.capemitgrey, nc);
+    if (rc) return rc;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+    };
+    HIPCHK(up(q.glev, glev, nlev * 8));
+    HIPCHK(up(q.wl0, wl0, nlin * 8)); HIPCHK(up(q.f, f, nlin * 8)); HIPCHK(up(q.gl, gl, nlin * 8));
+    HIPCHK(up(q.gu, gu, nlin * 8)); HIPCHK(up(q.aul, aul, nlin * 8)); HIPCHK(up(q.bul, bul, nlin * 8));
+    HIPCHK(up(q.blu, blu, nlin * 8));
+    if (ncoll > 0) {
+        HIPCHK(up(q.cc, coll_c, (size_t)4 * ncoll * 8)); HIPCHK(up(q.dele, dele, ncoll * 8));
+        HIPCHK(up(q.cl, llw_coll, ncoll * 4)); HIPCHK(up(q.cu, lup_coll, ncoll * 4));
+    }
+    HIPCHK(up(q.pic, pi_c, (size_t)4 * nlev * 8)); HIPCHK(up(q.rrc, rr_c, (size_t)4 * nlev * 8));
+    HIPCHK(up(q.delepi, dele_pi, nlev * 8));
+    HIPCHK(up(q.llw, llw, nlin * 4)); HIPCHK(up(q.lup, lup, nlin * 4)); HIPCHK(up(q.igs, igs.data(), nlin * 4));
+    HIPCHK(up(q.levpi, lev_pi, nlev * 4));
+    HIPCHK(up(q.hstart, hstart.data(), (ng + 2) * 4)); HIPCHK(up(q.hidx, hidx.data(), hidx.size() * 4));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_nlte = true;
+    return 0;
+}
+
+int slgpu_opacity_build_nlte(slgpu_ctx *c, const slgpu_gas *gas, const double *radtemp,
+                             const double *ndens, int flags, double *out_cap64, float *out_cap32,
+                             float *out_capemit, double *out_sig, double *out_capgrey,
+                             double *out_capemitgrey, double *out_capross, float *out_emiss,
+                             double *out_npop)
+{
+    int rc = check_opacity_inputs(c, gas);
+    if (rc) return rc;
+    if (!c->have_nlte) return fail(SLGPU_EORDER, "NLTE data must be set first");
+    if (!radtemp || !ndens) return fail(SLGPU_EINVAL, "null argument");
+    if (!(flags & SLGPU_NOBB))
+        for (int i = 0; i < c->nx; ++i) {
+            if (gas->mass[i] <= 0.0) continue;
+            if (!(radtemp[i] > 0.0) || !(ndens[2 * (size_t)i] > 0.0) || !(ndens[2 * (size_t)i + 1] >= 0.0))
+                return fail(SLGPU_EINVAL, "nlte opacity: radtemp and the H I density must be positive "
+                                          "and the H II density non-negative in every massive cell");
+        }
+    double *stage = nullptr;
+    rc = dalloc(c, &stage, (size_t)2 * std::max(c->nion, 1) * c->nx);
+    if (rc) return rc;
+    NlteCall nlc{radtemp, ndens, out_capemit, out_capemitgrey, out_npop};
+    rc = opacity_build_impl(c, gas, flags, out_cap64, out_cap32, out_sig, out_capgrey,
+                            out_capross, out_emiss, stage, &nlc);
+    hipStreamSynchronize(c->stream);
+    dfree(c, stage);
+    return rc;
 }
 
 int slgpu_ddmc_prep(slgpu_ctx *c, const double *radtemp, double lbol, double *out_opaclump,

@@ -224,6 +224,18 @@ __device__ __forceinline__ float exp_neg32(float x)
     return fmaf(p, e * 0.693147181f, p);
 }
 
+// Block-wide deterministic sum (fixed tree) for 256-thread blocks.
+__device__ __forceinline__ double block_sum256(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
 // order LDS writes of a wave before its later reads of them (each wave owns its slot)
 __device__ __forceinline__ void wave_lds_sync()
 {

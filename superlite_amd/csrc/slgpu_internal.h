@@ -104,6 +104,24 @@ struct Dev {
     double *eos_sahac2;                   // nc: Saha constant of the last evaluation (0: not solved)
 };
 
+// NLTE hydrogen (physical_opacity_nlte.f): the species data of slgpu_set_nlte_data, the
+// per-call inputs and the work arrays, passed by value to the NLTE kernels only.
+struct NlteDev {
+    int n = 0, nlin = 0, ncoll = 0, kH = -1;    // levels, lines, collision lines, H I ion index
+    double *glev, *wl0, *f, *gl, *gu, *aul, *bul, *blu;   // nlev / nlin
+    double *cc, *dele;                          // 4*ncoll, ncoll
+    double *pic, *rrc, *delepi;                 // 4*nlev, 4*nlev, nlev
+    int *llw, *lup, *igs;                       // nlin (igs: group of the line, 0 outside)
+    int *cl, *cu, *levpi;                       // ncoll, ncoll, nlev
+    int *hstart, *hidx;                         // ng+2 offsets, the lines by group in file order
+    double *radtemp, *ndens;                    // nc, 2*nc (H I, H II per cell)
+    double *rc, *A, *pop, *gpi;                 // nc*n*n, nc*n*n, nc*n, nc*n
+    int *ipiv, *flag;                           // nc*n, 1
+    double *capemitT, *emissT;                  // (ng, nc) cell-fastest
+    float *capemit32;                           // (nc, ng)
+    double *capemitgrey;                        // nc
+};
+
 struct Timer {
     hipEvent_t a, b;
     bool armed = false;
@@ -134,6 +152,8 @@ struct slgpu_ctx {
     double *eos_temp = nullptr, *eos_mass = nullptr, *eos_massfr = nullptr, *eos_nelec = nullptr,
            *eos_natom = nullptr, *eos_natomfr = nullptr, *eos_oc = nullptr, *eos_ginv = nullptr;
     int *eos_niter = nullptr;
+    slgpu::NlteDev nl;          // NLTE species data and work arrays (slgpu_set_nlte_data)
+    bool have_nlte = false;
     void *scratch = nullptr;    // device address of the library's static scratch array on this context's GPU
     double t_ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[24];
@@ -148,6 +168,9 @@ void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
                       int flags);
 void launch_opacity(slgpu_ctx *c, int flags);
 void launch_opacity_finish(slgpu_ctx *c);
+void launch_cell_tables(slgpu_ctx *c);
+void launch_opacity_nlte(slgpu_ctx *c, int flags, double ffc1, const double *rgg);
+void launch_nlte_means(slgpu_ctx *c);
 double ff_prefactor();
 const double *rgg_scratch(slgpu_ctx *c);
 void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double *gp,

@@ -300,17 +300,6 @@ __global__ __launch_bounds__(256) void k_opacity_finish(Dev d, int want64)
     }
 }
 
-// Block-wide deterministic sum (fixed tree) for 256-thread blocks.
-__device__ __forceinline__ double block_sum256(double v, double *sh)
-{
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    double r = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return r;
-}
 
 // Planck group integrals specintv(1/T) for the DDMC/emission tables, and the Planck and
 // Rosseland means of the FP32 group opacity.  One 256-thread block per cell.
@@ -400,6 +389,11 @@ void launch_opacity_finish(slgpu_ctx *c)
 {
     dim3 grid((c->ng + 31) / 32, (c->nx + 31) / 32);
     hipLaunchKernelGGL(k_opacity_finish, grid, dim3(256), 0, c->stream, c->d, 1);
+    launch_cell_tables(c);
+}
+
+void launch_cell_tables(slgpu_ctx *c)
+{
     hipLaunchKernelGGL(k_cell_tables, dim3(c->nx), dim3(256), 0, c->stream, c->d);
 }
 

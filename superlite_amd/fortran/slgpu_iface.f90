@@ -101,6 +101,28 @@ module slgpu_iface
       type(c_ptr), value :: cap64, cap32, sig, capgrey, capross, emiss
     end function
 
+    integer(c_int) function slgpu_set_nlte_data(ctx, nlev, nlin, ncoll, glev, llw, lup, wl0, f, &
+         gl, gu, aul, bul, blu, llw_coll, lup_coll, coll_c, dele, lev_pi, pi_c, rr_c, dele_pi) &
+         bind(C, name="slgpu_set_nlte_data")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nlev, nlin, ncoll
+      real(c_double), intent(in) :: glev(*), wl0(*), f(*), gl(*), gu(*), aul(*), bul(*), blu(*)
+      integer(c_int), intent(in) :: llw(*), lup(*), llw_coll(*), lup_coll(*), lev_pi(*)
+      real(c_double), intent(in) :: coll_c(*), dele(*), pi_c(*), rr_c(*), dele_pi(*)
+    end function
+
+    integer(c_int) function slgpu_opacity_build_nlte(ctx, gas, radtemp, ndens, flags, cap64, &
+         cap32, capemit, sig, capgrey, capemitgrey, capross, emiss, npop) &
+         bind(C, name="slgpu_opacity_build_nlte")
+      import :: c_int, c_ptr, c_double, slgpu_gas
+      type(c_ptr), value :: ctx
+      type(slgpu_gas), intent(in) :: gas
+      real(c_double), intent(in) :: radtemp(*), ndens(*)
+      integer(c_int), value :: flags
+      type(c_ptr), value :: cap64, cap32, capemit, sig, capgrey, capemitgrey, capross, emiss, npop
+    end function
+
     integer(c_int) function slgpu_ddmc_prep(ctx, radtemp, lbol, opaclump, emitprob, emit, &
          sthermal) bind(C, name="slgpu_ddmc_prep")
       import :: c_int, c_ptr, c_double

@@ -309,3 +309,121 @@ class OutputWriter:
                 self._write("output.grd_sig", _line_e(sig, expdigits=3))
             self._write("output.grd_eraddens", _line_e(np.asarray(tally, float) / np.asarray(vol, float)))
         self.first = False
+
+
+# ---------------------------------------------------------------------------
+# NLTE model atom (nltemod.f:88-166 read_atom_nlte, read_nlte_data.f:148-233)
+
+def _fixed(line: str, widths) -> list:
+    """Split one record by Fortran field widths (a TAB is one character, as a formatted
+    read counts it); negative widths are skipped columns (nX)."""
+    out, pos = [], 0
+    for w in widths:
+        if w < 0:
+            pos += -w
+            continue
+        out.append(line[pos:pos + w])
+        pos += w
+    return out
+
+
+def _num(s: str, kind=float):
+    s = s.strip()
+    if not s:
+        return kind(0)
+    return kind(float(s)) if kind is float else kind(s)
+
+
+def read_atom_nlte(path: str) -> dict:
+    """The raw records of a data.nlte.* file: levels '(i5,9x,i5,f11.3,i5)', radiative lines
+    '(i4,1x,i4,3x,es11.5,3x,es11.5)', electron-impact lines
+    '(i3,1x,i3,4(1x,es13.6),1x,f5.3,1x,es13.6)' and one PI/RR record per level
+    '(i3,2(4(1x,es13.6),1x,f5.3),1x,es13.6)' (RR fit first, then PI).  Stops as the
+    reference does on a bad header or on data left after the last record."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    head = lines[3].split()
+    nlev, nlin, ncoll = int(head[0]), int(head[1]), int(head[2])
+    if nlev <= 0 or nlin <= 0:
+        raise ValueError("read_atom_nlte: bad level or line count")
+    p = 4
+    lev = [_fixed(lines[p + k], (5, -9, 5, 11, 5)) for k in range(nlev)]
+    p += nlev
+    lin = [_fixed(lines[p + k], (4, -1, 4, -3, 11, -3, 11)) for k in range(nlin)]
+    p += nlin
+    col = [_fixed(lines[p + k], (3, -1, 3, -1, 13, -1, 13, -1, 13, -1, 13, -1, 5, -1, 13))
+           for k in range(ncoll)]
+    p += ncoll
+    pir = [_fixed(lines[p + k], (3,) + (-1, 13, -1, 13, -1, 13, -1, 13, -1, 5) * 2 + (-1, 13))
+           for k in range(nlev)]
+    p += nlev
+    if any(l.strip() for l in lines[p:]):
+        raise ValueError("read_atom_nlte: data remaining on input file")
+    r = {
+        "lev_id": np.array([_num(x[0], int) for x in lev]),
+        "lev_g": np.array([_num(x[1], int) for x in lev]),
+        "lev_chi": np.array([_num(x[2]) for x in lev]),
+        "lev_n": np.array([_num(x[3], int) for x in lev]),
+        "lin_lev1": np.array([_num(x[0], int) for x in lin]),
+        "lin_lev2": np.array([_num(x[1], int) for x in lin]),
+        "lin_f": np.array([_num(x[2]) for x in lin]),
+        "lin_wl0": np.array([_num(x[3]) for x in lin]),
+        "col_lev1": np.array([_num(x[0], int) for x in col]),
+        "col_lev2": np.array([_num(x[1], int) for x in col]),
+        "col_c": np.array([[_num(x[k]) for k in range(2, 6)] for x in col]).reshape(-1, 4),
+        "col_n": np.array([np.float32(_num(x[6])) for x in col], np.float32),
+        "col_dele": np.array([_num(x[7]) for x in col]),
+        "pi_lev": np.array([_num(x[0], int) for x in pir]),
+        "rr_c": np.array([[_num(x[k]) for k in range(1, 5)] for x in pir]),
+        "pi_c": np.array([[_num(x[k]) for k in range(6, 10)] for x in pir]),
+        "pi_dele": np.array([_num(x[11]) for x in pir]),
+    }
+    return r
+
+
+# physconstmod.f constants used by read_nlte_data.f
+_PC_PI = 3.1415926535897932385
+_PC_E = 4.80325e-10
+_PC_ME = 9.1093826e-28
+_PC_C = 2.997924562e10
+_PC_H = 6.6260693e-27
+_PC_ANG = 1e-8
+
+
+def nlte_species(raw: dict) -> dict:
+    """read_nlte_data.f:148-233 for one species: the permanent arrays the NLTE opacity uses.
+    Lines and collision lines are stored lower level first; a radiative line's wavelength
+    comes from its levels' energies (1e8/|chi_u|-|chi_l| in Angstrom) unless that is not
+    positive, then from the file; the Einstein coefficients use the file's wavelength and
+    Econst = 4 pi (pi e^2/(m_e c)) with the single-precision pi e^2/(m_e c) of the
+    reference."""
+    fconst = float(np.float32(_PC_PI * _PC_E ** 2 / (_PC_ME * _PC_C)))
+    econst = 4.0 * _PC_PI * fconst
+    chi, g = raw["lev_chi"], raw["lev_g"].astype(float)
+    lo = np.minimum(raw["lin_lev1"], raw["lin_lev2"])
+    hi = np.maximum(raw["lin_lev1"], raw["lin_lev2"])
+    n = len(lo)
+    wl0 = np.zeros(n)
+    for k in range(n):
+        dchi = abs(chi[hi[k] - 1]) - abs(chi[lo[k] - 1])
+        if dchi != 0.0:
+            wl0[k] = 1e8 / dchi
+        if wl0[k] <= 0.0:
+            wl0[k] = raw["lin_wl0"][k]
+    f = raw["lin_f"]
+    wf = raw["lin_wl0"] * _PC_ANG
+    gl, gu = g[lo - 1], g[hi - 1]
+    aul = (2.0 * econst * gl * f) / (gu * wf ** 2)
+    bul = (econst * wf * gl * f) / (_PC_H * _PC_C * gu)
+    blu = (econst * wf * f) / (_PC_H * _PC_C)
+    clo = np.minimum(raw["col_lev1"], raw["col_lev2"])
+    chi_ = np.maximum(raw["col_lev1"], raw["col_lev2"])
+    return {
+        "nlev": len(chi), "glev": g, "chilev": chi,
+        "llw": lo, "lup": hi, "wl0": wl0, "f": f, "gl": gl, "gu": gu,
+        "aul": aul, "bul": bul, "blu": blu,
+        "llw_coll": clo, "lup_coll": chi_, "coll_c": raw["col_c"], "dele": raw["col_dele"],
+        "lev_pi": raw["pi_lev"], "pi_c": raw["pi_c"], "rr_c": raw["rr_c"], "dele_pi": raw["pi_dele"],
+    }

@@ -29,7 +29,8 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_set_ions", "slgpu_opacity_build", "slgpu_ddmc_prep", "slgpu_sources",
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
            "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
-           "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp")
+           "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp",
+           "slgpu_set_nlte_data", "slgpu_opacity_build_nlte")
 
 
 class SlgpuError(RuntimeError):
@@ -71,6 +72,8 @@ def lib():
         L.slgpu_set_ions.argtypes = [_vp, _i, _vp, _vp]
         L.slgpu_opacity_build.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
         L.slgpu_opacity_build_noalloc.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
+        L.slgpu_set_nlte_data.argtypes = [_vp, _i, _i, _i] + [_vp] * 18
+        L.slgpu_opacity_build_nlte.argtypes = [_vp, ctypes.POINTER(Gas), _vp, _vp, _i] + [_vp] * 9
         L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
         L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
         L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _vp]
@@ -112,6 +115,7 @@ class Context:
         self.h = h
         self.npart = 0
         self.builds = 0          # counts opacity builds and EOS solves (state the backward uses)
+        self.nlev = 0            # levels of the NLTE atom (slgpu_set_nlte_data)
 
     def _check(self, rc):
         if rc != 0:
@@ -173,6 +177,45 @@ class Context:
         self._check(fn(self.h, ctypes.byref(gas), flags, _p(out.get("cap64")), _p(out.get("cap")),
                        _p(out.get("sig")), _p(out.get("capgrey")), _p(out.get("capross")),
                        _p(out.get("emiss"))))
+        return out
+
+    def set_nlte_data(self, sp):
+        """the H I model atom: a dict as superlite_amd.io.nlte_species returns"""
+        keep = dict(glev=_c(sp["glev"], np.float64), llw=_c(sp["llw"], np.int32),
+                    lup=_c(sp["lup"], np.int32), wl0=_c(sp["wl0"], np.float64),
+                    f=_c(sp["f"], np.float64), gl=_c(sp["gl"], np.float64), gu=_c(sp["gu"], np.float64),
+                    aul=_c(sp["aul"], np.float64), bul=_c(sp["bul"], np.float64),
+                    blu=_c(sp["blu"], np.float64), cl=_c(sp["llw_coll"], np.int32),
+                    cu=_c(sp["lup_coll"], np.int32), cc=_c(sp["coll_c"], np.float64),
+                    dele=_c(sp["dele"], np.float64), levpi=_c(sp["lev_pi"], np.int32),
+                    pic=_c(sp["pi_c"], np.float64), rrc=_c(sp["rr_c"], np.float64),
+                    delepi=_c(sp["dele_pi"], np.float64))
+        self.nlev = int(sp["nlev"])
+        self._check(self.L.slgpu_set_nlte_data(
+            self.h, self.nlev, len(keep["llw"]), len(keep["cl"]),
+            *[_p(keep[k]) for k in ("glev", "llw", "lup", "wl0", "f", "gl", "gu", "aul", "bul", "blu",
+                                    "cl", "cu", "cc", "dele", "levpi", "pic", "rrc", "delepi")]))
+
+    def opacity_build_nlte(self, temp, natom, vol, mass, nelec, natomfr, oc, ginv, radtemp, ndens,
+                           flags=0):
+        """physical_opacity_nlte + means: H I in NLTE.  ndens (nc, 2) H I and H II densities.
+        Returns cap64, cap, capemit, emiss (nc, ng), sig, capgrey, capemitgrey, capross (nc)
+        and npop (nc, nlev)."""
+        nc, ng = self.nx, self.ng
+        keep = [_c(temp, np.float64), _c(natom, np.float64), _c(vol, np.float64),
+                _c(mass, np.float64), _c(nelec, np.float64), _c(np.asarray(natomfr).T, np.float64),
+                _c(oc, np.float64), _c(ginv, np.float64)]
+        gas = Gas(*[_p(x) for x in keep])
+        rt, nd = _c(radtemp, np.float64), _c(ndens, np.float64)
+        out = dict(cap64=np.zeros((nc, ng)), cap=np.zeros((nc, ng), np.float32),
+                   capemit=np.zeros((nc, ng), np.float32), emiss=np.zeros((nc, ng), np.float32),
+                   sig=np.zeros(nc), capgrey=np.zeros(nc), capemitgrey=np.zeros(nc),
+                   capross=np.zeros(nc), npop=np.zeros((nc, max(self.nlev, 1))))
+        self.builds += 1
+        self._check(self.L.slgpu_opacity_build_nlte(
+            self.h, ctypes.byref(gas), _p(rt), _p(nd), flags, _p(out["cap64"]), _p(out["cap"]),
+            _p(out["capemit"]), _p(out["sig"]), _p(out["capgrey"]), _p(out["capemitgrey"]),
+            _p(out["capross"]), _p(out["emiss"]), _p(out["npop"])))
         return out
 
     def ddmc_prep(self, radtemp, lbol, want=True):

@@ -11,6 +11,8 @@ Outputs
   eos_<w>.npz           the reference's LTE EOS data for workload <w> (ni, ionisation energies,
                         partition functions frozen at its first EOS call, ground-level weights,
                         element masses)
+  nlte_h1.npz           the H I model atom Data/Atoms/data.nlte.h1, its records as the
+                        reference's read_atom_nlte reads them (superlite_amd.io.read_atom_nlte)
   state_<w>.npz         host state of each workload (grid, shell masses/temperatures, and the
                         ionisation balance from the reference's LTE EOS, eos_update.f:23-45)
   (in tests/golden/, the fixtures)
@@ -34,7 +36,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-from oracle.pyref import RefSession  # noqa: E402
+from oracle.pyref import REF_DATA, RefSession  # noqa: E402
 from superlite_amd import synth  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -79,7 +81,16 @@ def main():
     ap.add_argument("--mc-modes", action="store_true",
                     help="only the c1 reference Monte Carlo with the amplification factor and "
                          "anisotropic Thomson scattering on (ref_c1modes_mc.npz)")
+    ap.add_argument("--nlte-data", action="store_true",
+                    help="only the H I model atom superlite_amd/data/nlte_h1.npz, the records of "
+                         "the reference's Data/Atoms/data.nlte.h1 as read_atom_nlte reads them")
     args = ap.parse_args()
+    if args.nlte_data:
+        from superlite_amd import io
+        raw = io.read_atom_nlte(os.path.join(REF_DATA, "Atoms", "data.nlte.h1"))
+        np.savez_compressed(os.path.join(DATA, "nlte_h1.npz"), **raw)
+        print("nlte data", {k: np.asarray(v).shape for k, v in raw.items()}, flush=True)
+        return
     if args.mc_c2:
         mc_case("c2", 1.0, 10, 8, workload="c2")
         return
