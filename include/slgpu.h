@@ -185,12 +185,21 @@ int slgpu_set_nlte_data(slgpu_ctx *ctx, int nlev, int nlin, int ncoll, const dou
  * densities of eos_update.f:52-57.  Outputs as slgpu_opacity_build, plus out_capemit
  * (gas_capemit, ng*nc FP32), out_capemitgrey (nc) and out_npop (nlev*nc, level-fastest, the
  * normalised populations); any may be NULL.  FP64 arithmetic in both precision forms.
- * SLGPU_EINVAL where the reference stops on a singular rate matrix. */
+ * SLGPU_EINVAL where the reference stops on a singular rate matrix.  After this build the
+ * DDMC prep and the transport's group samplers take the emission opacity from gas_capemit and
+ * gas_capemitgrey, as the reference's in_nlte branches do (leakage_opacity11.f90:76-85,
+ * emission_probability.f90:28-60, emitgroup.f:23-50, diffusion11.f90:136-182,576-581); after an
+ * LTE build they take gas_cap and gas_capgrey. */
 int slgpu_opacity_build_nlte(slgpu_ctx *ctx, const slgpu_gas *gas, const double *radtemp,
                              const double *ndens, int flags, double *out_cap64, float *out_cap32,
                              float *out_capemit, double *out_sig, double *out_capgrey,
                              double *out_capemitgrey, double *out_capross, float *out_emiss,
                              double *out_npop);
+
+/* nltemod.f:231-260 nlte_rad_est, after slgpu_transport: from the transport's census energy
+ * tally grd_tally, radtemp (in/out, nc) += 0.5 ((tally/(a vol))^(1/4) - radtemp), and out_jrad
+ * (ng*nc group-fastest, may be NULL) = wlm^2 B(wlm, radtemp)/c at each group centre. */
+int slgpu_nlte_rad_est(slgpu_ctx *ctx, double *radtemp, double *out_jrad);
 
 /* Differentiation form: vector-Jacobian product of the last slgpu_opacity_build.
  * Backward of physical_opacity.f:93-247 (bb, bf, ff), opacity_planckmean.f:18-24 and
@@ -202,7 +211,8 @@ int slgpu_opacity_build_nlte(slgpu_ctx *ctx, const slgpu_gas *gas, const double 
  * FP32-rounded cap as in the forward, the rounding taken as the identity.  Density and
  * composition enter the opacity through the EOS populations (eos_update.f:37-45), which
  * the host computes: dL/drho and dL/dX follow by chaining these outputs through it.  Line
- * terms are recomputed, not stored.  Needs nion + (elements present) <= 128. */
+ * terms are recomputed, not stored.  Needs nion + (elements present) <= 128.
+ * SLGPU_EORDER after an NLTE build: the backward covers the LTE build. */
 int slgpu_opacity_vjp(slgpu_ctx *ctx, const double *grad_cap64, const double *grad_capgrey,
                       const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
                       double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc);

@@ -698,8 +698,12 @@ int or_opacity_nlte(const or_opac_in *in, const or_nlte_in *nl, double *cap64, f
 #define WLINV(ig) (m->wlinv[(ig) - 1])
 #define DIVV(ix) (m->divv[(ix) - 1])
 #define TINV(ic) (m->tempinv[(ic) - 1])
+/* NLTE emission opacity (in_nlte) */
+#define NLTE (m->capemit != NULL)
+#define CAPE(ig, ic) ((double)m->capemit[((ic) - 1) * (size_t)m->ng + ((ig) - 1)])
 
-/* GRID/leakage_opacity11.f90:30-171 (LTE: capemit == 0, so opaclump(9) == 0) */
+/* GRID/leakage_opacity11.f90:30-171 (LTE: capemit == 0, so opaclump(9) == 0 and the capemit
+ * loop never moves igemitmax) */
 void or_leakage(const or_model *m, double *opaclump)
 {
     const int ng = m->ng, nx = m->nx;
@@ -727,6 +731,16 @@ void or_leakage(const or_model *m, double *opaclump)
             caplump += help;
             if (help > emitmax) { emitmax = help; igemitmax = ig; }
         }
+        /* leakage_opacity11.f90:76-85 capemitlump (NLTE) */
+        double capemitlump = 0.0;
+        if (NLTE)
+            for (int ig = 1; ig <= ng; ++ig) {
+                if (CAP(ig, l) * dist < m->taulump) continue;
+                if ((SIG(l) + CAP(ig, l)) * dist < m->tauddmc) continue;
+                double help = specarr[ig - 1] * CAPE(ig, l);
+                capemitlump += help;
+                if (help > emitmax) { emitmax = help; igemitmax = ig; }
+            }
         double doplump = 0.0;
         if (DIVV(l) >= 0) {
             for (int ig = 1; ig <= ng - 1; ++ig) {
@@ -744,7 +758,7 @@ void or_leakage(const or_model *m, double *opaclump)
             }
         }
         op[7] = caplump;
-        op[8] = 0.0;
+        op[8] = capemitlump;
         op[9] = (double)igemitmax;
         op[10] = doplump;
         double dvol = 3.0 * m->vol[l - 1] / PC_PI4;
@@ -775,7 +789,7 @@ void or_leakage(const or_model *m, double *opaclump)
     free(specarr);
 }
 
-/* GRID/emission_probability.f90:39-62 (LTE, grp_ng > 1) */
+/* GRID/emission_probability.f90:39-62 (grp_ng > 1; grd_capemit and grd_capemitgrey in NLTE) */
 void or_emission_probability(const or_model *m, double *emitprob)
 {
     const int ng = m->ng;
@@ -789,12 +803,12 @@ void or_emission_probability(const or_model *m, double *emitprob)
             int nepg = hi - ig + 1;
             int igp1 = ig + nepg - 1;
             double s = 0.0;
-            for (int g = ig; g <= igp1; ++g) s += specarr[g - 1] * CAP(g, l);
+            for (int g = ig; g <= igp1; ++g) s += specarr[g - 1] * (NLTE ? CAPE(g, l) : CAP(g, l));
             help = help + s;
             emitprob[(size_t)(l - 1) * m->nep + iep - 1] = help;
             ig = igp1 + 1;
         }
-        emitprob[(size_t)(l - 1) * m->nep + m->nep - 1] = m->capgrey[l - 1];
+        emitprob[(size_t)(l - 1) * m->nep + m->nep - 1] = NLTE ? m->capemitgrey[l - 1] : m->capgrey[l - 1];
     }
     free(specarr);
 }
@@ -803,7 +817,7 @@ void or_emission_probability(const or_model *m, double *emitprob)
 int or_emitgroup(const or_model *m, double r, int ic)
 {
     const double *ep = m->emitprob + (size_t)(ic - 1) * m->nep;
-    double r1 = r * m->capgrey[ic - 1];
+    double r1 = r * (NLTE ? m->capemitgrey[ic - 1] : m->capgrey[ic - 1]);
     int iep = or_binsrch(r1, ep, m->nep, 1);
     int ig = iep * m->nepg + 1;
     int igp1 = ig + m->nepg - 1 < m->ng ? ig + m->nepg - 1 : m->ng;
@@ -814,7 +828,7 @@ int or_emitgroup(const or_model *m, double r, int ic)
     int l = 0;
     for (; ig <= igp1 - 1; ++ig) {
         l = l + 1;
-        emitprob = emitprob + specval[l - 1] * CAP(ig, ic);
+        emitprob = emitprob + specval[l - 1] * (NLTE ? CAPE(ig, ic) : CAP(ig, ic));
         if (emitprob > r1) break;
     }
     return ig;
@@ -939,6 +953,21 @@ int or_eos(int nc, const or_eos_data *ed, const double *temp, const double *mass
 
 /* ------------------------------------------------------------------------ */
 /* SOURCE/sourceenergy.f90:23-50 (surface-like source) */
+/* nltemod.f:231-260 */
+void or_nlte_rad_est(int nc, int ng, const double *wl, const double *vol, const double *tally,
+                     double *radtemp, double *jrad)
+{
+    for (int i = 0; i < nc; ++i) {
+        double trad = pow(tally[i] / (PC_ACOEF * vol[i]), 0.25);
+        radtemp[i] = radtemp[i] + 0.5 * (trad - radtemp[i]);
+        if (!jrad) continue;
+        for (int ig = 1; ig <= ng; ++ig) {
+            double wlm = 0.5 * (wl[ig] + wl[ig - 1]);
+            jrad[(size_t)i * ng + ig - 1] = wlm * wlm * or_planck(wlm, radtemp[i]) / PC_C;
+        }
+    }
+}
+
 void or_sourceenergy(int nc, const double *rcell, const double *radtemp, double lbol,
                      double *emit, double *sthermal)
 {
@@ -1087,7 +1116,7 @@ typedef struct {
     int *glumps;        /* ng */
     unsigned char *llumps;
     int ic, nlump;
-    double capgreyinv, speclump, emitlump, caplump, doplump;
+    double capgreyinv, capemitgreyinv, speclump, emitlump, caplump, doplump;
 } or_cache;
 
 /* diffusion11.f90:95-152: cell quantities (recomputed whenever the cell changes) */
@@ -1099,6 +1128,11 @@ static void or_cache_cell(const or_model *m, or_cache *c, int ic, int ix)
     or_specintv(TINV(ic), m->wlinv, ng, 1, c->specarr);
     c->capgreyinv = 1.0 / m->capgrey[ic - 1];
     if (!(c->capgreyinv >= 0.0)) c->capgreyinv = 0.0;
+    c->capemitgreyinv = 0.0;
+    if (NLTE) {
+        c->capemitgreyinv = 1.0 / m->capemitgrey[ic - 1];
+        if (!(c->capemitgreyinv >= 0.0)) c->capemitgreyinv = 0.0;
+    }
     int glump = 0, gunlump = ng;
     for (int k = 0; k < ng; ++k) c->glumps[k] = 0;
     c->speclump = OPL(7, ic);
@@ -1121,7 +1155,7 @@ static void or_cache_cell(const or_model *m, or_cache *c, int ic, int ix)
         c->doplump = 0.0;
     } else {
         c->caplump = OPL(8, ic) * c->speclump;
-        double el = OPL(8, ic) * c->capgreyinv;
+        double el = NLTE ? OPL(9, ic) * c->capemitgreyinv : OPL(8, ic) * c->capgreyinv;
         c->emitlump = el < 1.0 ? el : 1.0;
         c->doplump = OPL(11, ic) * c->speclump;
     }
@@ -1449,7 +1483,8 @@ static int or_diffusion11(const or_model *m, or_ptcl *p, double *vx, or_cache *c
         caplump = c->caplump;
         doplump = c->doplump;
     } else {
-        emitlump = or_specint0(TINV(ic), m->wlinv, ig) * c->capgreyinv * CAP(ig, ic);
+        emitlump = NLTE ? or_specint0(TINV(ic), m->wlinv, ig) * c->capemitgreyinv * CAPE(ig, ic)
+                        : or_specint0(TINV(ic), m->wlinv, ig) * c->capgreyinv * CAP(ig, ic);
         caplump = CAP(ig, ic);
         if (ig != ng && DIVV(ix) >= 0)
             doplump = or_dopspeccalc(TINV(ic), m->wlinv, ig) * DIVV(ix) / (c->specarr[ig - 1] * 3 * PC_C);
@@ -1648,7 +1683,8 @@ static int or_diffusion11(const or_model *m, or_ptcl *p, double *vx, or_cache *c
             double denom3 = 0.0;
             for (int iig = ng; iig >= glump + 1; --iig) {
                 iiig = c->glumps[iig - 1];
-                double help = c->specarr[iiig - 1] * CAP(iiig, ic) * c->capgreyinv;
+                double help = NLTE ? c->specarr[iiig - 1] * CAPE(iiig, ic) * c->capemitgreyinv
+                                   : c->specarr[iiig - 1] * CAP(iiig, ic) * c->capgreyinv;
                 denom3 = denom3 + help * denom2;
                 if (denom3 > r1) break;
             }

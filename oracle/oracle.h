@@ -47,6 +47,11 @@ typedef struct {
     const double *emit;           /* nc, after sourceenergy */
     const double *radtemp;        /* nc */
     const int *nvol;              /* nc, after sourcenumbers */
+    /* in_nlte: the emission opacity grd_capemit (ng*nc) and grd_capemitgrey (nc) that the
+     * NLTE branches of leakage_opacity11, emission_probability, emitgroup and diffusion11
+     * read; both NULL for LTE */
+    const float *capemit;
+    const double *capemitgrey;
 } or_model;
 
 typedef struct {
@@ -163,6 +168,11 @@ int or_lu_solve(int n, double *a, int *ipiv, double *b);
 void or_leakage(const or_model *m, double *opaclump);
 void or_emission_probability(const or_model *m, double *emitprob);
 int or_emitgroup(const or_model *m, double r, int ic);                       /* GRID/emitgroup.f:3-54 */
+
+/* nltemod.f:231-260 nlte_rad_est: radtemp (in/out, nc) from the tally grd_tally (nc) and the
+ * cell volumes; jrad (ng*nc group-fastest) at the group centres of wl (ng+1) */
+void or_nlte_rad_est(int nc, int ng, const double *wl, const double *vol, const double *tally,
+                     double *radtemp, double *jrad);
 
 /* ---- sources: sourceenergy.f90:23-50, sourcenumbers.f90:26-169 (nmpi=1, no surface source) ---- */
 void or_sourceenergy(int nc, const double *rcell, const double *radtemp, double lbol,

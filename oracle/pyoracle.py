@@ -41,6 +41,7 @@ def lib():
         _lib.or_sourcenumbers.restype = ctypes.c_int64
         _lib.or_sourcenumbers.argtypes = [_i, _vp, ctypes.c_int64, _vp]
         _lib.or_sourceenergy.argtypes = [_i, _vp, _vp, _d, _vp, _vp]
+        _lib.or_nlte_rad_est.argtypes = [_i, _i, _vp, _vp, _vp, _vp, _vp]
         _lib.or_iteration.restype = _i
         _lib.or_iteration.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp]
         _lib.or_sources.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32] + [_vp] * 6
@@ -81,7 +82,7 @@ class _Model(ctypes.Structure):
                 ("ampfact", _i), ("thm_aniso", _i),
                 ("cap", _vp), ("sig", _vp), ("capgrey", _vp), ("tempinv", _vp),
                 ("opaclump", _vp), ("emitprob", _vp), ("emit", _vp), ("radtemp", _vp),
-                ("nvol", _vp)]
+                ("nvol", _vp), ("capemit", _vp), ("capemitgrey", _vp)]
 
 
 class _Result(ctypes.Structure):
@@ -274,7 +275,9 @@ class Model:
     def __init__(self, xarr, vxarr, divv, vol, wl, nep, nepg, cap, sig, capgrey, tempinv,
                  radtemp, tauddmc=5.0, taulump=10.0, puretran=False, max_steps=0,
                  opaclump=None, emitprob=None, emit=None, nvol=None, ampfact=False,
-                 thm_aniso=False):
+                 thm_aniso=False, capemit=None, capemitgrey=None):
+        """capemit (nc, ng) FP32 and capemitgrey (nc): the NLTE emission opacity (in_nlte);
+        None for LTE."""
         self.a = dict(xarr=_c(xarr, np.float64), vxarr=_c(vxarr, np.float64),
                       divv=_c(divv, np.float64), vol=_c(vol, np.float64),
                       wl=_c(wl, np.float64), wlinv=_c(1.0 / np.asarray(wl, np.float64), np.float64),
@@ -289,10 +292,14 @@ class Model:
         for k, v in (("opaclump", opaclump), ("emitprob", emitprob), ("emit", emit)):
             self.a[k] = None if v is None else _c(v, np.float64)
         self.a["nvol"] = None if nvol is None else _c(nvol, np.int32)
+        if (capemit is None) != (capemitgrey is None):
+            raise ValueError("capemit and capemitgrey go together")
+        self.a["capemit"] = None if capemit is None else _c(capemit, np.float32)
+        self.a["capemitgrey"] = None if capemitgrey is None else _c(capemitgrey, np.float64)
 
     def set(self, **kw):
         for k, v in kw.items():
-            dt = np.int32 if k == "nvol" else (np.float32 if k == "cap" else np.float64)
+            dt = np.int32 if k == "nvol" else (np.float32 if k in ("cap", "capemit") else np.float64)
             self.a[k] = _c(v, dt)
 
     def struct(self):
@@ -302,7 +309,7 @@ class Model:
                       self.ng, _p(a["wl"]), _p(a["wlinv"]), self.nep, self.nepg, t[0], t[1],
                       t[2], t[3], t[4], t[5], _p(a["cap"]), _p(a["sig"]), _p(a["capgrey"]),
                       _p(a["tempinv"]), _p(a["opaclump"]), _p(a["emitprob"]), _p(a["emit"]),
-                      _p(a["radtemp"]), _p(a["nvol"]))
+                      _p(a["radtemp"]), _p(a["nvol"]), _p(a["capemit"]), _p(a["capemitgrey"]))
 
     def leakage(self):
         out = np.zeros((self.nx, 11))
@@ -351,6 +358,16 @@ class Model:
         if per_particle:
             out["particles"] = pp
         return out
+
+
+def nlte_rad_est(wl, vol, tally, radtemp):
+    """nltemod.f:231-260: the updated radtemp (nc) and jrad (nc, ng)"""
+    wl, vol, tally = _c(wl, np.float64), _c(vol, np.float64), _c(tally, np.float64)
+    rt = np.array(radtemp, np.float64)
+    nc, ng = len(vol), len(wl) - 1
+    jrad = np.zeros((nc, ng))
+    lib().or_nlte_rad_est(nc, ng, _p(wl), _p(vol), _p(tally), _p(rt), _p(jrad))
+    return rt, jrad
 
 
 def sourceenergy(rcell, radtemp, lbol):

@@ -225,6 +225,21 @@ class RefSession:
         self.lib.slref_get_prep(_i(self.nc), _i(self.nep), _p(opaclump), _p(emitprob), _p(emit))
         return dict(opaclump=opaclump, emitprob=emitprob, emit=emit)
 
+    def set_nlte(self, capemit=None, capemitgrey=None):
+        """in_nlte with the emission opacity capemit (nc, ng) FP32 and capemitgrey (nc) for the
+        following prep and transport; no arguments: back to LTE"""
+        on = capemit is not None
+        c = np.ascontiguousarray(capemit if on else np.zeros((self.nc, self.ng)), np.float32)
+        g = np.ascontiguousarray(capemitgrey if on else np.zeros(self.nc), np.float64)
+        self.lib.slref_set_nlte(_i(int(on)), _i(self.nc), _i(self.ng), _p(c), _p(g))
+
+    def nlte_rad_est(self, tally, radtemp):
+        t = np.ascontiguousarray(tally, np.float64)
+        rt = np.array(radtemp, np.float64)
+        jrad = np.zeros((self.nc, self.ng))
+        self.lib.slref_nlte_rad_est(_i(self.nc), _i(self.ng), _p(t), _p(rt), _p(jrad))
+        return rt, jrad
+
     def emitgroup(self, r, ic):
         return self.lib.slref_emitgroup(_d(r), _i(ic))
 

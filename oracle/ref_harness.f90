@@ -305,15 +305,56 @@ contains
   !   (superlite.f90:163-178)
   subroutine slref_prep() bind(C, name="slref_prep")
     use gasmod, only: gas_emit, gas_capemit, gas_capemitgrey, gas_jrad
+    use inputparmod, only: in_nlte
     use mpimod, only: bcast_nonpermanent
     gas_emit = 0d0
-    gas_capemit = 0.
-    gas_capemitgrey = 0d0
+    if(.not.in_nlte) then
+      gas_capemit = 0.
+      gas_capemitgrey = 0d0
+    endif
     gas_jrad = 0d0
     call bcast_nonpermanent
     call sourceenergy
     call leakage_opacity
     call emission_probability
+  end subroutine
+
+  !-- in_nlte and the NLTE emission opacity gas_capemit / gas_capemitgrey that the next
+  !   slref_prep broadcasts (flag 0: back to LTE)
+  subroutine slref_set_nlte(flag, nc, ng, capemit, capemitgrey) bind(C, name="slref_set_nlte")
+    use gasmod, only: gas_capemit, gas_capemitgrey
+    use inputparmod, only: in_nlte
+    integer(c_int), value :: flag, nc, ng
+    real(c_float), intent(in) :: capemit(ng, nc)
+    real(c_double), intent(in) :: capemitgrey(nc)
+    in_nlte = flag /= 0
+    if(in_nlte) then
+      gas_capemit = capemit
+      gas_capemitgrey = capemitgrey
+    endif
+  end subroutine
+
+  !-- nlte_rad_est (superlite.f90:208) on a given grd_tally and grd_radtemp; the module's
+  !   grd_tally, grd_radtemp and grd_jrad are restored afterwards
+  subroutine slref_nlte_rad_est(nc, ng, tally, radtemp, jrad) bind(C, name="slref_nlte_rad_est")
+    use gridmod, only: grd_tally, grd_radtemp, grd_jrad
+    use nltemod, only: nlte_rad_est
+    integer(c_int), value :: nc, ng
+    real(c_double), intent(in) :: tally(nc)
+    real(c_double), intent(inout) :: radtemp(nc)
+    real(c_double), intent(out) :: jrad(ng, nc)
+    real(c_double) :: t0(nc), r0(nc), j0(ng, nc)
+    t0 = grd_tally
+    r0 = grd_radtemp
+    j0 = grd_jrad
+    grd_tally = tally
+    grd_radtemp = radtemp
+    call nlte_rad_est
+    radtemp = grd_radtemp
+    jrad = grd_jrad
+    grd_tally = t0
+    grd_radtemp = r0
+    grd_jrad = j0
   end subroutine
 
   subroutine slref_get_prep(nc, nep, opaclump, emitprob, emit) bind(C, name="slref_get_prep")

@@ -222,6 +222,9 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.emiss, cg);
     rc |= dalloc(c, &d.capgrey, nc);
     rc |= dalloc(c, &d.capross, nc);
+    d.capem32 = d.cap32;
+    d.capemgrey = d.capgrey;
+    d.nlte = 0;
     rc |= dalloc(c, &d.specarr, cg);
     rc |= dalloc(c, &d.opaclump, 11 * nc);
     rc |= dalloc(c, &d.emitprob, (size_t)cfg->nep * nc);
@@ -594,7 +597,11 @@ int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *ou
     if (!rc && nlc) {
         int flag = 0;
         HIPCHK(hipMemcpy(&flag, c->nl.flag, sizeof(int), hipMemcpyDeviceToHost));
-        if (flag) return fail(SLGPU_EINVAL, "nlte opacity: rate matrix numerically singular");
+        if (flag) {
+            c->have_opacity = false;        // the device opacity is partly overwritten
+            c->have_prep = false;
+            return fail(SLGPU_EINVAL, "nlte opacity: rate matrix numerically singular");
+        }
     }
     float kms = 0.f;
     hipEventElapsedTime(&kms, c->ev[10], c->ev[11]);
@@ -602,9 +609,17 @@ int opacity_build_impl(slgpu_ctx *c, const slgpu_gas *gas, int flags, double *ou
     hipEventElapsedTime(&kms, c->ev[22], c->ev[23]);
     c->t_ms[11] += kms;
     c->opac_flags = flags;
-    if (rc) return rc;
-    c->have_opacity = true;
+    // the emission opacity the DDMC prep and the samplers read (gridmod.f:41,56 grd_capemit,
+    // grd_capemitgrey; the absorption opacity in LTE)
+    c->d.nlte = nlc ? 1 : 0;
+    c->d.capem32 = nlc ? c->nl.capemit32 : c->d.cap32;
+    c->d.capemgrey = nlc ? c->nl.capemitgrey : c->d.capgrey;
     c->have_prep = false;
+    if (rc) {
+        c->have_opacity = false;
+        return rc;
+    }
+    c->have_opacity = true;
     return 0;
 }
 
@@ -720,6 +735,13 @@ int slgpu_set_nlte_data(slgpu_ctx *c, int nlev, int nlin, int ncoll, const doubl
                       q.capemit32, q.capemitgrey};
         for (void *p : ps) dfree(c, p);
         c->have_nlte = false;
+        if (c->d.nlte) {            // the last build's emission opacity is gone with them
+            c->d.nlte = 0;
+            c->d.capem32 = c->d.cap32;
+            c->d.capemgrey = c->d.capgrey;
+            c->have_opacity = false;
+            c->have_prep = false;
+        }
     }
     q = NlteDev();
     q.n = nlev; q.nlin = nlin; q.ncoll = ncoll; q.kH = kH;
@@ -788,6 +810,29 @@ int slgpu_opacity_build_nlte(slgpu_ctx *c, const slgpu_gas *gas, const double *r
                             out_capross, out_emiss, stage, &nlc);
     hipStreamSynchronize(c->stream);
     dfree(c, stage);
+    return rc;
+}
+
+int slgpu_nlte_rad_est(slgpu_ctx *c, double *radtemp, double *out_jrad)
+{
+    if (!c || !radtemp) return fail(SLGPU_EINVAL, "null argument");
+    if (!c->have_transport) return fail(SLGPU_EORDER, "slgpu_transport must run first");
+    const int nc = c->nx, ng = c->ng;
+    for (int i = 0; i < nc; ++i)
+        if (!(radtemp[i] > 0.0)) return fail(SLGPU_EINVAL, "grd_radtemp must be positive");
+    double *drt = nullptr, *dj = nullptr;
+    int rc = dalloc(c, &drt, nc);
+    if (!rc && out_jrad) rc = dalloc(c, &dj, (size_t)nc * ng);
+    if (!rc) {
+        HIPCHK(hipMemcpyAsync(drt, radtemp, nc * 8, hipMemcpyHostToDevice, c->stream));
+        launch_nlte_rad_est(c, drt, dj);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(radtemp, drt, nc * 8, hipMemcpyDeviceToHost, c->stream));
+        if (dj) HIPCHK(hipMemcpyAsync(out_jrad, dj, (size_t)nc * ng * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (drt) dfree(c, drt);
+    if (dj) dfree(c, dj);
     return rc;
 }
 
@@ -939,6 +984,7 @@ int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad
 {
     if (!c) return fail(SLGPU_EINVAL, "null context");
     if (!c->have_opacity) return fail(SLGPU_EORDER, "slgpu_opacity_build must run first");
+    if (c->d.nlte) return fail(SLGPU_EORDER, "opacity backward: the last build was NLTE; the backward covers the LTE build");
     const int nc = c->nx, ng = c->ng, nion = c->nion;
     if ((size_t)(nion + c->d.nel) * 64 * sizeof(double) > 65536)
         return fail(SLGPU_EINVAL, "opacity backward: ions + elements exceed 128 per cell");

@@ -100,6 +100,8 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 }
 
 #define CAPC(ig, ic) ((double)d.cap32[(size_t)((ic) - 1) * ng + ((ig) - 1)])
+// emission opacity: grd_capemit after an NLTE build, grd_cap after an LTE one
+#define CAPE(ig, ic) ((double)d.capem32[(size_t)((ic) - 1) * ng + ((ig) - 1)])
 #define XA(i) (d.xarr[(i) - 1])
 #define DX(i) (d.xarr[(i)] - d.xarr[(i) - 1])
 #define DX3(i) (d.xarr[(i)] * d.xarr[(i)] * d.xarr[(i)] - d.xarr[(i) - 1] * d.xarr[(i) - 1] * d.xarr[(i) - 1])
@@ -135,6 +137,31 @@ __device__ __forceinline__ double resleak(const Dev &d, int ix, int g, int side)
     return 2.0 * (XA(ix + 1) * XA(ix + 1)) / (mfp * DX3(ix));
 }
 
+// first index of the block's maximum of (v, k) pairs, k = 0 meaning none (the strict '>' of
+// an ascending loop): the result is valid on thread 0
+__device__ __forceinline__ void block_argmax_first(double &v, int &k, double *s_v, int *s_k)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        double v2 = __shfl_xor(v, off, 64);
+        int k2 = __shfl_xor(k, off, 64);
+        if (v2 > v || (v2 == v && k2 != 0 && (k == 0 || k2 < k))) { v = v2; k = k2; }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_v[w] = v; s_k[w] = k; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bv = 0.0;
+        int bk = 0;
+        for (int q = 0; q < NW; ++q)
+            if (s_v[q] > bv || (s_v[q] == bv && s_k[q] != 0 && (bk == 0 || s_k[q] < bk))) {
+                bv = s_v[q]; bk = s_k[q];
+            }
+        v = bv;
+        k = bk;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(NT) void k_leakage(Dev d)
 {
     __shared__ double shd[NW];
@@ -160,9 +187,9 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
     int nlump;
     block_scan_int(nl, shi, nlump);
     const double speclump = sl > 0.0 ? 1.0 / sl : 0.0;
-    // pass 2: caplump, igemitmax, doplump, leakage opacities (:61-168)
-    double caplump = 0.0, doplump = 0.0, o1 = 0.0, o2 = 0.0, emax = 0.0;
-    int imax = 0;
+    // pass 2: caplump, capemitlump (NLTE), igemitmax, doplump, leakage opacities (:61-168)
+    double caplump = 0.0, capemitlump = 0.0, doplump = 0.0, o1 = 0.0, o2 = 0.0, emax = 0.0, emax2 = 0.0;
+    int imax = 0, imax2 = 0;
     const double dvol = 3.0 * d.vol[l - 1] / PC_PI4;
     for (int g = threadIdx.x + 1; g <= ng; g += NT) {
         const double c = CAPC(g, l);
@@ -171,6 +198,11 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
             const double h = spec[g - 1] * c;
             caplump += h;
             if (h > emax) { emax = h; imax = g; }
+            if (d.nlte) {
+                const double he = spec[g - 1] * CAPE(g, l);
+                capemitlump += he;
+                if (he > emax2) { emax2 = he; imax2 = g; }
+            }
             const double specval = spec[g - 1];
             bool lh = (i == 1) ? true : ((CAPC(g, icnb1) + SIGC(icnb1)) * DX(i - 1) < d.tauddmc);
             if (lh) {
@@ -203,31 +235,19 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
     doplump = block_sum(doplump, shd);
     o1 = block_sum(o1, shd);
     o2 = block_sum(o2, shd);
-    // first index of the maximum (strict '>' in the reference's ascending loop)
-    {
-        double v = emax;
-        int k = imax;
-        for (int off = 32; off > 0; off >>= 1) {
-            double v2 = __shfl_xor(v, off, 64);
-            int k2 = __shfl_xor(k, off, 64);
-            if (v2 > v || (v2 == v && k2 != 0 && (k == 0 || k2 < k))) { v = v2; k = k2; }
-        }
-        const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) { s_emax[w] = v; s_imax[w] = k; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double bv = 0.0;
-            int bk = 0;
-            for (int q = 0; q < NW; ++q)
-                if (s_emax[q] > bv || (s_emax[q] == bv && s_imax[q] != 0 && (bk == 0 || s_imax[q] < bk))) {
-                    bv = s_emax[q]; bk = s_imax[q];
-                }
-            imax = bk;
-        }
+    // igemitmax: first index of the maximum (strict '>' in the reference's ascending loop);
+    // in NLTE the capemit loop continues from the cap loop's maximum, so its first maximum
+    // wins only where it is strictly larger
+    block_argmax_first(emax, imax, s_emax, s_imax);
+    if (d.nlte) {
+        capemitlump = block_sum(capemitlump, shd);
+        block_argmax_first(emax2, imax2, s_emax, s_imax);
+        if (emax2 > emax) imax = imax2;
     }
     double *op = d.opaclump + (size_t)(l - 1) * 11;
     const double capgrey = d.capgrey[l - 1];
-    double cgi = 1.0 / capgrey;
+    // capgreyinv (LTE) or capemitgreyinv (NLTE), diffusion11.f90:98-99
+    double cgi = 1.0 / d.capemgrey[l - 1];
     if (!(cgi >= 0.0)) cgi = 0.0;
     const int nl_eff = speclump == 0.0 ? 0 : nlump;
     double c_emitlump, c_caplump, c_doplump;
@@ -235,15 +255,16 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
         c_emitlump = 1.0; c_caplump = capgrey; c_doplump = 0.0;
     } else {
         c_caplump = caplump * speclump;
-        double el = caplump * cgi;
+        double el = (d.nlte ? capemitlump : caplump) * cgi;     // diffusion11.f90:136-141
         c_emitlump = el < 1.0 ? el : 1.0;
         c_doplump = doplump * speclump;
     }
     if (threadIdx.x == 0) {
         op[0] = o1; op[1] = o2; op[2] = 0.0; op[3] = 0.0; op[4] = 0.0; op[5] = 0.0;
-        op[6] = speclump; op[7] = caplump; op[8] = 0.0; op[9] = (double)imax; op[10] = doplump;
+        op[6] = speclump; op[7] = caplump; op[8] = d.nlte ? capemitlump : 0.0;
+        op[9] = (double)imax; op[10] = doplump;
         CellDDMC cc;
-        cc.capgreyinv = cgi; cc.speclump = speclump; cc.emitlump = c_emitlump;
+        cc.emgreyinv = cgi; cc.speclump = speclump; cc.emitlump = c_emitlump;
         cc.caplump = c_caplump; cc.doplump = c_doplump; cc.nlump = nl_eff; cc.pad = 0;
         d.cell[l - 1] = cc;
     }
@@ -296,7 +317,7 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
     }
     for (int p = nl_eff + threadIdx.x; p < ng; p += NT) {
         const int g = gorder[p];
-        cS[p] = spec[g - 1] * CAPC(g, l) * cgi * d2s;
+        cS[p] = spec[g - 1] * CAPE(g, l) * cgi * d2s;     // diffusion11.f90:574-581
     }
     __syncthreads();
     if (threadIdx.x == 0) running_sum(cL, nl_eff);
@@ -305,7 +326,8 @@ __global__ __launch_bounds__(NT) void k_leakage(Dev d)
     else if (threadIdx.x == 192) running_sum(cS + nl_eff, ng - nl_eff);
 }
 
-// emission_probability.f90:39-62 and the in-bin cumulative sums emitgroup.f:278-294 walks
+// emission_probability.f90:39-62 (grd_capemit, grd_capemitgrey in NLTE) and the in-bin
+// cumulative sums emitgroup.f:38-50 walks
 __global__ __launch_bounds__(256) void k_emitprob(Dev d)
 {
     __shared__ double binsum[1024];
@@ -317,7 +339,7 @@ __global__ __launch_bounds__(256) void k_emitprob(Dev d)
         const int g0 = b * nepg + 1;
         const int g1 = (b + 1) * nepg < ng ? (b + 1) * nepg : ng;
         double s = 0.0;
-        for (int g = g0; g <= g1; ++g) s += spec[g - 1] * CAPC(g, l);
+        for (int g = g0; g <= g1; ++g) s += spec[g - 1] * CAPE(g, l);
         binsum[b] = s;
     }
     __syncthreads();
@@ -330,7 +352,7 @@ __global__ __launch_bounds__(256) void k_emitprob(Dev d)
             ep[iep - 1] = help;
             pref[iep] = help;
         }
-        ep[nep - 1] = d.capgrey[l - 1];
+        ep[nep - 1] = d.capemgrey[l - 1];
     }
     __syncthreads();
     double *cum = d.emitcum + (size_t)(l - 1) * ng;
@@ -339,7 +361,7 @@ __global__ __launch_bounds__(256) void k_emitprob(Dev d)
         const int g1 = (b + 1) * nepg < ng ? (b + 1) * nepg : ng;
         double s = pref[b];
         for (int g = g0; g <= g1; ++g) {
-            s = s + spec[g - 1] * CAPC(g, l);
+            s = s + spec[g - 1] * CAPE(g, l);
             cum[g - 1] = s;
         }
     }

@@ -21,7 +21,8 @@ struct LineRec {
 
 // Per-cell DDMC constants (diffusion11.f90:95-152), evaluated once per iteration.
 struct CellDDMC {
-    double capgreyinv, speclump, emitlump, caplump, doplump;
+    double emgreyinv;   // 1/capgrey (LTE) or 1/capemitgrey (NLTE), nan -> 0
+    double speclump, emitlump, caplump, doplump;
     int nlump;
     int pad;
 };
@@ -60,6 +61,11 @@ struct Dev {
     double *cap64;              // nc*ng group-fastest
     float *cap32, *emiss;       // nc*ng group-fastest (gas_cap, gas_emiss)
     double *capgrey, *capross;
+    // emission opacity of the DDMC prep and the emission samplers: grd_capemit and
+    // grd_capemitgrey after an NLTE build (nlte = 1), cap32 and capgrey after an LTE one
+    const float *capem32;
+    const double *capemgrey;
+    int nlte;
     // DDMC prep
     double *specarr;            // nc*ng  specintv(tempinv(ic)) group-fastest
     double *opaclump;           // 11*nc
@@ -166,6 +172,7 @@ void launch_bfxs(slgpu_ctx *c, const float *d_ph1, const float *d_ph2, const int
 void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
                       const double *d_nelec, const double *d_oc, const double *d_ginv,
                       int flags);
+void launch_nlte_rad_est(slgpu_ctx *c, double *radtemp, double *jrad);
 void launch_opacity(slgpu_ctx *c, int flags);
 void launch_opacity_finish(slgpu_ctx *c);
 void launch_cell_tables(slgpu_ctx *c);

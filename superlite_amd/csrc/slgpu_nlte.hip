@@ -14,6 +14,8 @@
 //                  emiss as the reference adds them
 //   k_nlte_finish  transpose to the (nc, ng) outputs; k_nlte_means the Planck mean of capemit
 //                  (opacity_planckmean.f:22)
+//   k_nlte_rad_est the radiation temperature and intensity estimates after the transport
+//                  (nltemod.f:231-260)
 //
 // The reference inverts the rate matrix (dgetrf + dgetri) and multiplies; the populations
 // here solve the same system with the same factorisation.  The arithmetic is FP64 in both
@@ -325,6 +327,34 @@ __global__ __launch_bounds__(256) void k_nlte_means(Dev d, NlteDev q)
         pe += (double)q.capemit32[(size_t)i * ng + g - 1] * d.specarr[(size_t)i * ng + g - 1];
     pe = block_sum256(pe, sh);
     if (threadIdx.x == 0) q.capemitgrey[i] = pe;
+}
+
+// nlte_rad_est (nltemod.f:231-260) after the transport: the radiation temperature estimate
+// (tally/(a V))^(1/4) from the census energy tally, damped by one half, and the group-centre
+// intensity wlm^2 B(wlm, radtemp)/c at the new radiation temperature; one block per cell
+__global__ __launch_bounds__(256) void k_nlte_rad_est(Dev d, double *radtemp, double *jrad)
+{
+    __shared__ double s_tr;
+    const int i = blockIdx.x, ng = d.ng;
+    if (threadIdx.x == 0) {
+        const double trad = pow(d.tally[i] / (PC_ACOEF * d.vol[i]), 0.25);
+        const double r = radtemp[i];
+        const double rn = r + 0.5 * (trad - r);
+        radtemp[i] = rn;
+        s_tr = rn;
+    }
+    __syncthreads();
+    if (!jrad) return;
+    const double tr = s_tr;
+    for (int g = threadIdx.x + 1; g <= ng; g += 256) {
+        const double wlm = 0.5 * (d.wl[g] + d.wl[g - 1]);
+        jrad[(size_t)i * ng + g - 1] = wlm * wlm * planck(wlm, tr) / PC_C;
+    }
+}
+
+void launch_nlte_rad_est(slgpu_ctx *c, double *radtemp, double *jrad)
+{
+    hipLaunchKernelGGL(k_nlte_rad_est, dim3(c->nx), dim3(256), 0, c->stream, c->d, radtemp, jrad);
 }
 
 void launch_opacity_nlte(slgpu_ctx *c, int flags, double ffc1, const double *rgg)

@@ -90,12 +90,12 @@ __device__ __forceinline__ int first_above(const double *__restrict__ cum, int l
 }
 
 // GRID/emitgroup.f:3-54 by direct search: binsrch over the binned emission probabilities,
-// then the first group of the bin whose running sum exceeds r*capgrey
+// then the first group of the bin whose running sum exceeds r*capgrey (r*capemitgrey in NLTE)
 __device__ __forceinline__ int emitgroup_direct(const Dev &d, double r, int ic)
 {
     const int ng = d.ng;
     const double *ep = d.emitprob + (size_t)(ic - 1) * d.nep;
-    const double r1 = r * d.capgrey[ic - 1];
+    const double r1 = r * d.capemgrey[ic - 1];
     int iep = binsrch(r1, ep, d.nep, true);
     if (iep > d.nep - 1) iep = d.nep - 1;
     if (iep < 0) iep = 0;
@@ -112,7 +112,7 @@ __device__ __forceinline__ int emitgroup_range(const Dev &d, double r, int ic, i
 {
     const int ng = d.ng, nepg = d.nepg;
     const double *ep = d.emitprob + (size_t)(ic - 1) * d.nep;
-    const double r1 = r * d.capgrey[ic - 1];
+    const double r1 = r * d.capemgrey[ic - 1];
     int iep = (glo - 1) / nepg;
     const int iep_hi = (ghi - 1) / nepg;
     while (iep < iep_hi && ep[iep] <= r1) ++iep;

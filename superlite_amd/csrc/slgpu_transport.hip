@@ -482,7 +482,9 @@ __global__ __launch_bounds__(256) void k_ddmc_probs(Dev d)
     if (ig > ng) return;
     const CellDDMC cc = d.cell[ic - 1];
     const double cp = CAPT(ig, ic);
-    const double emitlump = specint0(TINV(ic), d.wlinv, ig) * cc.capgreyinv * cp;
+    // diffusion11.f90:175-182: grd_cap(ig)*capgreyinv in LTE, grd_capemit(ig)*capemitgreyinv in NLTE
+    const double ce = (double)d.capem32[(size_t)(ic - 1) * ng + ig - 1];
+    const double emitlump = specint0(TINV(ic), d.wlinv, ig) * cc.emgreyinv * ce;
     const double sa = d.specarr[(size_t)(ic - 1) * ng + ig - 1];
     double doplump;
     if (ig != ng && DIVV(ix) >= 0)

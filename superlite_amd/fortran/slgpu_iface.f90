@@ -123,6 +123,15 @@ module slgpu_iface
       type(c_ptr), value :: cap64, cap32, capemit, sig, capgrey, capemitgrey, capross, emiss, npop
     end function
 
+    ! nlte_rad_est (nltemod.f:231-260): call after slgpu_transport with grd_radtemp (in/out);
+    ! jrad (ng*nc) may be c_null_ptr
+    integer(c_int) function slgpu_nlte_rad_est(ctx, radtemp, jrad) bind(C, name="slgpu_nlte_rad_est")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: radtemp(*)
+      type(c_ptr), value :: jrad
+    end function
+
     integer(c_int) function slgpu_ddmc_prep(ctx, radtemp, lbol, opaclump, emitprob, emit, &
          sthermal) bind(C, name="slgpu_ddmc_prep")
       import :: c_int, c_ptr, c_double

@@ -30,7 +30,7 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
            "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
            "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp",
-           "slgpu_set_nlte_data", "slgpu_opacity_build_nlte")
+           "slgpu_set_nlte_data", "slgpu_opacity_build_nlte", "slgpu_nlte_rad_est")
 
 
 class SlgpuError(RuntimeError):
@@ -74,6 +74,7 @@ def lib():
         L.slgpu_opacity_build_noalloc.argtypes = [_vp, ctypes.POINTER(Gas), _i] + [_vp] * 6
         L.slgpu_set_nlte_data.argtypes = [_vp, _i, _i, _i] + [_vp] * 18
         L.slgpu_opacity_build_nlte.argtypes = [_vp, ctypes.POINTER(Gas), _vp, _vp, _i] + [_vp] * 9
+        L.slgpu_nlte_rad_est.argtypes = [_vp, _vp, _vp]
         L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
         L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
         L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _vp]
@@ -217,6 +218,14 @@ class Context:
             _p(out["capemit"]), _p(out["sig"]), _p(out["capgrey"]), _p(out["capemitgrey"]),
             _p(out["capross"]), _p(out["emiss"]), _p(out["npop"])))
         return out
+
+    def nlte_rad_est(self, radtemp, want_jrad=True):
+        """nlte_rad_est after the transport: the updated radtemp (nc) and the group-centre
+        intensity jrad (nc, ng), or None without want_jrad."""
+        rt = np.array(radtemp, np.float64)
+        jrad = np.zeros((self.nx, self.ng)) if want_jrad else None
+        self._check(self.L.slgpu_nlte_rad_est(self.h, _p(rt), _p(jrad)))
+        return rt, jrad
 
     def ddmc_prep(self, radtemp, lbol, want=True):
         nc = self.nx
@@ -372,3 +381,18 @@ def iteration(ctx, st, nstot, it, lbol, want=True, flags=0):
     tr = ctx.transport(it, want=want)
     fl = ctx.fluxtally() if want else ctx.fluxtally()
     return dict(opacity=op, prep=prep, nvol=nvol, nnew=nnew, transport=tr, flux=fl)
+
+
+def iteration_nlte(ctx, st, nstot, it, lbol, radtemp, ndens, flags=0):
+    """One iteration with H I in NLTE (superlite.f90:157-208 with in_nlte): the NLTE opacity
+    build, the DDMC prep and samplers on gas_capemit, sources, transport, flux tally, and
+    nlte_rad_est.  Returns the tallies and the updated radtemp."""
+    op = ctx.opacity_build_nlte(st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"],
+                                st["natomfr"], st["oc"], st["ginv"], radtemp, ndens, flags=flags)
+    prep = ctx.ddmc_prep(radtemp, lbol)
+    nvol, nnew = ctx.sources(nstot, it)
+    tr = ctx.transport(it)
+    fl = ctx.fluxtally()
+    rt, jrad = ctx.nlte_rad_est(radtemp)
+    return dict(opacity=op, prep=prep, nvol=nvol, nnew=nnew, transport=tr, flux=fl,
+                radtemp=rt, jrad=jrad)

@@ -22,6 +22,10 @@ Outputs
                         (independent realisations, superlite.f90:148-219) for statistical tests
   ref_edge_opacity.npz  a hand-built 3-cell / 5-group case with lines on, below and above the
                         group edges, and the reference's FP32 opacities for it
+  ref_c1nlte_mc.npz     the transport with in_nlte on workload c1: a given emission opacity
+                        gas_capemit / gas_capemitgrey beside the LTE gas_cap, the reference's
+                        NLTE grd_opaclump, grd_emitprob and emitgroup answers, its Monte Carlo
+                        tallies, and nlte_rad_est on the first iteration's tally
 
 Usage:  python tests/golden/make_golden.py [--mc] [--mc-c2]
 """
@@ -81,6 +85,8 @@ def main():
     ap.add_argument("--mc-modes", action="store_true",
                     help="only the c1 reference Monte Carlo with the amplification factor and "
                          "anisotropic Thomson scattering on (ref_c1modes_mc.npz)")
+    ap.add_argument("--mc-nlte", action="store_true",
+                    help="the reference's transport with in_nlte on c1 (ref_c1nlte_mc.npz)")
     ap.add_argument("--nlte-data", action="store_true",
                     help="only the H I model atom superlite_amd/data/nlte_h1.npz, the records of "
                          "the reference's Data/Atoms/data.nlte.h1 as read_atom_nlte reads them")
@@ -113,6 +119,9 @@ def main():
     if args.io:
         deck_cases()
         output_cases()
+        return
+    if args.mc_nlte:
+        mc_case("c1nlte", 1.0, args.mc_n2s, args.mc_iter, nlte=True)
         return
     if args.mc_modes:
         mc_case("c1modes", 1.0, args.mc_n2s, args.mc_iter, ampfact=True, thm_aniso=True)
@@ -420,7 +429,22 @@ def sourcenumbers_cases():
           sum(r["first"] < r["ns"] for r in rows), sum(r["first"] > r["ns"] for r in rows), flush=True)
 
 
-def mc_case(case, scale, n2s, niter, workload="c1", ampfact=False, thm_aniso=False):
+def nlte_emission_opacity(cap, temp, spec):
+    """A stand-in NLTE emission opacity for the transport coupling: the absorption opacity
+    scaled by a smooth factor in [e^-1.2, e^1.2] that varies over groups and cells, zero in
+    groups 601-640 (no emission there), and its Planck mean (opacity_planckmean.f:22)
+    over spec(nc, ng) = specintv(1/T)."""
+    nc, ng = cap.shape
+    g = np.arange(1, ng + 1)[None, :]
+    c = np.arange(1, nc + 1)[:, None]
+    f = np.exp(1.2 * np.sin(2 * np.pi * (3.0 * g / ng + c / 17.0)))
+    capemit = (cap.astype(np.float64) * f).astype(np.float32)
+    capemit[:, 600:640] = 0.0
+    capemitgrey = np.array([np.sum(capemit[i].astype(np.float64) * spec[i]) for i in range(nc)])
+    return capemit, capemitgrey
+
+
+def mc_case(case, scale, n2s, niter, workload="c1", ampfact=False, thm_aniso=False, nlte=False):
     """Reference transport statistics: niter independent iterations of 2^n2s particles."""
     w = synth.WORKLOADS[workload]
     st = dict(np.load(os.path.join(DATA, f"state_{workload}.npz")))
@@ -444,7 +468,19 @@ def mc_case(case, scale, n2s, niter, workload="c1", ampfact=False, thm_aniso=Fal
                         st["natomfr"], st["ion_iz"], st["ion_ii"], st["oc"] * scale, st["ginv"],
                         lines, synth.load_tables(), flags=0 if scale == 1.0 else 4)
         s.put_opacity(op["cap"], op["sig"], op["capgrey"], op["capross"])
+    extra_nlte = {}
+    if nlte:
+        spec = np.array([s.specintv(1.0 / t, w.ng) for t in st["temp"]])
+        capemit, capemitgrey = nlte_emission_opacity(op["cap"], st["temp"], spec)
+        s.set_nlte(capemit, capemitgrey)
+        extra_nlte = dict(capemit=capemit, capemitgrey=capemitgrey)
     p = s.prep()
+    if nlte:
+        rng = np.random.default_rng(17)
+        eg_r = rng.random(4000)
+        eg_ic = rng.integers(1, w.nx + 1, 4000).astype(np.int32)
+        eg_ig = np.array([s.emitgroup(r, int(ic)) for r, ic in zip(eg_r, eg_ic)], np.int32)
+        extra_nlte.update(eg_r=eg_r, eg_ic=eg_ic, eg_ig=eg_ig)
     keys = ("luminos", "lumnum", "tally", "eamp", "eout", "evelo", "nstepimc", "nstepddmc",
             "nmethswap", "nflux")
     rows = {k: [] for k in keys}
@@ -453,6 +489,12 @@ def mc_case(case, scale, n2s, niter, workload="c1", ampfact=False, thm_aniso=Fal
         for k in keys:
             rows[k].append(r[k])
         print(case, "iteration", it, "eout", r["eout"], "steps", r["nstepimc"], r["nstepddmc"], flush=True)
+        if nlte and it == 0:
+            # nlte_rad_est (superlite.f90:208) on this tally from a radtemp of 0.9 T
+            rt_in = 0.9 * st["temp"]
+            rt_out, jrad_out = s.nlte_rad_est(r["tally"], rt_in)
+            extra_nlte.update(re_tally=r["tally"].copy(), re_radtemp_in=rt_in,
+                              re_radtemp=rt_out, re_jrad=jrad_out)
         # written after every iteration, so a long run keeps what it has finished; the c1
         # files also keep the iteration's opacity and DDMC tables (small at c1)
         extra = {}
@@ -461,7 +503,7 @@ def mc_case(case, scale, n2s, niter, workload="c1", ampfact=False, thm_aniso=Fal
                          opaclump=p["opaclump"], emitprob=p["emitprob"], emit=p["emit"])
         np.savez_compressed(os.path.join(OUT, f"ref_{case}_mc.npz"), n2s=n2s, scale=scale,
                             workload=workload, flags=0 if scale == 1.0 else 4,
-                            ampfact=int(ampfact), thm_aniso=int(thm_aniso), **extra,
+                            ampfact=int(ampfact), thm_aniso=int(thm_aniso), **extra, **extra_nlte,
                             **{k: np.array(v) for k, v in rows.items()})
     s.close()
 

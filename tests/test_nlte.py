@@ -247,3 +247,111 @@ def test_nlte_lte_parts_match_lte_build(c2_cells):
     np.testing.assert_allclose(o["capgrey"], lte["capgrey"], rtol=1e-12)
     np.testing.assert_allclose(o["capemitgrey"], o["capgrey"], rtol=1e-15)
     np.testing.assert_allclose(o["capross"], lte["capross"], rtol=1e-12)
+
+
+# --- the transport with in_nlte (leakage_opacity11.f90:76-108, emission_probability.f90:28-60,
+# emitgroup.f:23-50, diffusion11.f90:136-182,576-581, nltemod.f:231-260) against the
+# reference's own Fortran with in_nlte on (tests/golden/ref_c1nlte_mc.npz).  The emission
+# opacity there is a stand-in for gas_capemit (the absorption opacity scaled by a factor that
+# varies over groups and cells, zero in groups 601-640): workload c1 has no hydrogen, and the
+# coupling depends on the arrays alone, not on how they were made.
+@pytest.fixture(scope="module")
+def c1nlte(c1, golden):
+    w, st, wl, lines = c1
+    g = golden("ref_c1nlte_mc.npz")
+    return w, st, wl, g
+
+
+def _zstats(ref_rows, our_rows):
+    """Welch z of the per-column means of two sets of independent realisations"""
+    a, b = np.asarray(ref_rows, float), np.asarray(our_rows, float)
+    ma, sa = a.mean(0), a.std(0, ddof=1) / np.sqrt(len(a))
+    mb, sb = b.mean(0), b.std(0, ddof=1) / np.sqrt(len(b))
+    s = np.sqrt(sa ** 2 + sb ** 2)
+    ok = s > 0
+    return (ma[ok] - mb[ok]) / s[ok]
+
+
+def coarse(a, nb=20):
+    a = np.asarray(a)
+    n = a.shape[-1] - a.shape[-1] % nb
+    return a[..., :n].reshape(a.shape[:-1] + (nb, -1)).sum(-1)
+
+
+def _nlte_model(st, w, wl, g, **kw):
+    return po.Model(st["xarr"], st["vxarr"], st["divv"], st["vol"], wl, int(st["nep"]),
+                    int(st["nepg"]), g["cap"], g["sig"], g["capgrey"], 1.0 / st["temp"],
+                    st["temp"], tauddmc=w.tauddmc, taulump=w.taulump, capemit=g["capemit"],
+                    capemitgrey=g["capemitgrey"], **kw)
+
+
+def test_nlte_prep_vs_reference(c1nlte, golden):
+    """grd_opaclump with opaclump(9) = capemitlump and igemitmax continued over the capemit
+    loop, and grd_emitprob on capemit, equal the reference's to round-off; the stand-in
+    changes igemitmax in at least one cell, so the continuation is exercised"""
+    w, st, wl, g = c1nlte
+    m = _nlte_model(st, w, wl, g)
+    ol = m.leakage()
+    for k in (0, 1, 6, 7, 8, 10):
+        np.testing.assert_allclose(ol[:, k], g["opaclump"][:, k], rtol=1e-13, atol=0, err_msg=str(k))
+    assert np.array_equal(ol[:, 9], g["opaclump"][:, 9])
+    assert np.all(g["opaclump"][:, 8] > 0)
+    lte = golden("ref_c1_mc.npz")
+    assert np.count_nonzero(lte["opaclump"][:, 9] != g["opaclump"][:, 9]) >= 1
+    np.testing.assert_allclose(m.emission_probability(), g["emitprob"], rtol=1e-13)
+    # the oracle without capemit is the LTE prep: opaclump(9) = 0
+    m0 = po.Model(st["xarr"], st["vxarr"], st["divv"], st["vol"], wl, int(st["nep"]),
+                  int(st["nepg"]), g["cap"], g["sig"], g["capgrey"], 1.0 / st["temp"], st["temp"],
+                  tauddmc=w.tauddmc, taulump=w.taulump)
+    assert np.all(m0.leakage()[:, 8] == 0)
+
+
+def test_nlte_emitgroup_vs_reference(c1nlte):
+    """emitgroup on capemit: every answer equals the reference's, and no group without
+    emission (601-640) is drawn"""
+    w, st, wl, g = c1nlte
+    m = _nlte_model(st, w, wl, g, opaclump=g["opaclump"], emitprob=g["emitprob"], emit=g["emit"])
+    got = np.array([m.emitgroup(r, int(ic)) for r, ic in zip(g["eg_r"], g["eg_ic"])])
+    assert np.array_equal(got, g["eg_ig"])
+    assert not np.any((got > 600) & (got <= 640))
+
+
+def test_nlte_rad_est_vs_reference(c1nlte):
+    """nlte_rad_est: radtemp to 1 ulp-level, jrad to round-off; and the closed form of a
+    tally at radiative equilibrium with radtemp (a fixed point)"""
+    w, st, wl, g = c1nlte
+    rt, jrad = po.nlte_rad_est(wl, st["vol"], g["re_tally"], g["re_radtemp_in"])
+    np.testing.assert_allclose(rt, g["re_radtemp"], rtol=4e-16, atol=0)
+    np.testing.assert_allclose(jrad, g["re_jrad"], rtol=1e-14, atol=0)
+    t = np.linspace(5e3, 5e4, len(st["vol"]))
+    tally = 7.5657e-15 * st["vol"] * t ** 4
+    rt2, j2 = po.nlte_rad_est(wl, st["vol"], tally, t)
+    np.testing.assert_allclose(rt2, t, rtol=1e-15)
+    wlm = 0.5 * (wl[1:] + wl[:-1])
+    x = PC_H * PC_C / (PC_KB * wlm[None, :] * t[:, None])
+    b = 2 * PC_H * PC_C ** 2 / wlm[None, :] ** 5 / np.expm1(np.minimum(x, 300.0))
+    np.testing.assert_allclose(j2, wlm[None, :] ** 2 * b / PC_C, rtol=1e-12)
+
+
+def test_oracle_nlte_transport_vs_reference_statistics(c1nlte, golden):
+    """the oracle's transport with in_nlte (16 iterations of 2^14 particles) against 24 of the
+    reference's: luminosity per group, tally per shell, e_out, e_velo and the IMC and DDMC step
+    counts; and the same reference rows against its LTE ones differ, so the test can tell"""
+    w, st, wl, g = c1nlte
+    m = _nlte_model(st, w, wl, g, opaclump=g["opaclump"], emitprob=g["emitprob"], emit=g["emit"],
+                    nvol=po.sourcenumbers(g["emit"], 1 << int(g["n2s"])))
+    rows = [m.iteration(seed=23, iteration=it + 1) for it in range(16)]
+    for r in rows:
+        assert r["estart"] == pytest.approx(r["eout"] + r["evelo"] + r["edead"], rel=1e-12)
+        assert r["nerr"] == 0
+    for key, f in (("luminos", coarse), ("tally", lambda a: coarse(a, 10))):
+        z = _zstats(f(g[key]), [f(r[key]) for r in rows])
+        assert np.mean(z ** 2) < 2.5, (key, np.round(z, 2))
+        assert np.max(np.abs(z)) < 4.5, (key, np.round(z, 2))
+    for key in ("eout", "evelo", "nstepimc", "nstepddmc"):
+        z = _zstats(g[key][:, None], [[r[key]] for r in rows])
+        assert abs(z[0]) < 4.0, (key, z)
+    lte = golden("ref_c1_mc.npz")
+    for key in ("eout", "nstepddmc"):
+        z = _zstats(g[key][:, None], lte[key][:, None])
+        assert abs(z[0]) > 10.0, (key, z)
