@@ -228,6 +228,69 @@ def opacity_grad(args):
         "forward_cell_groups_per_s": cg / (fwd * 1e-3)}))
 
 
+def nlte_iteration(args):
+    """Iterations with hydrogen in NLTE (superlite.f90:157-208 with in_nlte), every step on the
+    device: the EOS and its nlte_ndens, the NLTE opacity build (per shell the 100-level H I
+    rate equations and their LU solve, the H lines' NLTE absorption and emission beside the
+    other ions' LTE lines, bf and ff, the means), the DDMC prep and samplers on the emission
+    opacity, sources, transport, flux tally and nlte_rad_est.  Every iteration starts from the
+    same gas state and radiation temperature (0.9 T), so each times the same work; value =
+    particle-steps per second of whole-iteration wall time, as in the LTE transport mode."""
+    from superlite_amd import slgpu, synth
+    w, st, wl, lines = synth.build(args.workload)
+    tables = synth.load_tables()
+    prec = slgpu.SLGPU_FP64 if args.precision == "fp64" else slgpu.SLGPU_MIXED
+    nstot = 1 << w.n2s
+    ctx = slgpu.setup_workload(w, st, wl, lines, tables, precision=prec, seed=2024,
+                               npartmax=nstot, errorfatal=False)
+    ctx.set_eos_data(synth.load_eos_data(args.workload))
+    ctx.set_nlte_data(synth.load_nlte_data())
+    radtemp = 0.9 * st["temp"]
+
+    def step(n, it):
+        e = ctx.eos(st["temp"], st["mass"], st["massfr"], nelec=st["nelec"])
+        nd = ctx.eos_nlte_ndens()
+        s2 = dict(st)
+        s2.update(natom=e["natom"], natomfr=e["natomfr"], nelec=e["nelec"], oc=e["oc"], ginv=e["ginv"])
+        return slgpu.iteration_nlte(ctx, s2, n, it, w.lbol, radtemp, nd)
+    for k in range(args.warmup):
+        step(1 << min(args.warmup_n2s, w.n2s), 1000 + k)
+    ctx.timers(reset=True)
+    steps, nerr, done = 0, 0, 0
+    t0 = time.perf_counter()
+    while done < args.steps:
+        r = step(nstot, done + 1)
+        tr = r["transport"]
+        steps += tr["nstepimc"] + tr["nstepddmc"]
+        nerr += tr["nerr"]
+        done += 1
+        el = time.perf_counter() - t0
+        print(f"# nlte step {done - 1}: particle-steps {tr['nstepimc'] + tr['nstepddmc']}, escaped "
+              f"{tr['nflux']}, errors {tr['nerr']}, elapsed {el:.1f} s", file=sys.stderr, flush=True)
+        if el + el / done > args.budget_s:
+            break
+    dt = time.perf_counter() - t0
+    tm = ctx.timers()
+    ctx.close()
+    K = done
+    mixed = prec == slgpu.SLGPU_MIXED
+    print(json.dumps({
+        "metric": "particle-steps/s, one iteration with H I in NLTE",
+        "value": steps / dt, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": K, "steps_requested": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {w.nx} shells x {w.ng} groups x {w.nline} lines, "
+                               f"2^{w.n2s} particles/iteration, H I in NLTE (100 levels, 900 lines)",
+                   "precision": "mixed" if mixed else "fp64",
+                   "step": "EOS + nlte_ndens, NLTE opacity, DDMC prep, sources, transport, "
+                           "flux tally, nlte_rad_est"},
+        "transport_errors": nerr,
+        "nlte_opacity_kernels_ms": tm["opacity_kernels"] / K,
+        "nlte_opacity_cell_groups_per_s": w.nx * w.ng * K / (tm["opacity_kernels"] * 1e-3),
+        "phase_ms_per_step": {k: v / K for k, v in tm.items()}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,8 +306,9 @@ def main():
     ap.add_argument("--warmup-n2s", type=int, default=16,
                     help="log2 particles of the warmup iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="transport", choices=["transport", "opacity-grad"],
-                    help="opacity-grad: time the opacity build and its backward instead")
+    ap.add_argument("--mode", default="transport", choices=["transport", "opacity-grad", "nlte"],
+                    help="opacity-grad: time the opacity build and its backward instead; "
+                         "nlte: whole iterations with H I in NLTE (precision mixed unless fp64)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one elementwise copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -255,6 +319,9 @@ def main():
 
     if args.mode == "opacity-grad":
         opacity_grad(args)
+        return
+    if args.mode == "nlte":
+        nlte_iteration(args)
         return
     if args.calibration_copy_mib > 0:
         calibration_copy(args.calibration_copy_mib)

@@ -237,6 +237,12 @@ int slgpu_eos(slgpu_ctx *ctx, const double *temp, const double *mass, const doub
               double acc, double *nelec, double *out_natom, double *out_natomfr,
               double *out_oc, double *out_ginv, int *out_niter);
 
+/* eos_update.f:47-58 with in_nlte (nlte_nelem = 1): nlte_ndens(i,1:2,1) of the last slgpu_eos,
+ * the H I and H II number densities n_ii natom natomfr(1)/vol from its last Saha evaluation.
+ * out_ndens (2*nc, H I then H II per cell; 0 in massless cells) is the ndens argument of
+ * slgpu_opacity_build_nlte. */
+int slgpu_eos_nlte_ndens(slgpu_ctx *ctx, double *out_ndens);
+
 /* Differentiation form: vector-Jacobian product of the last slgpu_eos at its solution (the
  * implicit function theorem on the converged nelec, partition functions held fixed).
  * dL/d(natom (nc), natomfr (30*nc), nelec (nc), oc (nion*nc)) in, any may be NULL (zero);

@@ -1122,6 +1122,22 @@ int slgpu_eos(slgpu_ctx *c, const double *temp, const double *mass, const double
     return 0;
 }
 
+int slgpu_eos_nlte_ndens(slgpu_ctx *c, double *out_ndens)
+{
+    if (!c || !out_ndens) return fail(SLGPU_EINVAL, "null argument");
+    if (!c->have_eos) return fail(SLGPU_EORDER, "slgpu_eos must run first");
+    const int nc = c->nx;
+    double *dn = nullptr;
+    int rc = dalloc(c, &dn, (size_t)2 * nc);
+    if (rc) return rc;
+    launch_eos_ndens(c, c->eos_temp, c->eos_mass, c->eos_natom, c->eos_natomfr, dn);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out_ndens, dn, (size_t)2 * nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, dn);
+    return 0;
+}
+
 int slgpu_eos_vjp(slgpu_ctx *c, const double *grad_natom, const double *grad_natomfr,
                   const double *grad_nelec, const double *grad_oc, double *out_grad_temp,
                   double *out_grad_mass, double *out_grad_massfr)

@@ -5,6 +5,7 @@
 //              fractions, the Saha/secant solve for the electron fraction and the ground-level
 //              populations of the context's ion list
 //   k_eos_vjp  the adjoint of k_eos at its converged solution
+//   k_eos_ndens the H I / H II number densities of the last solve (eos_update.f:47-58)
 //
 // The work is tiny (at most 8192 cells, each ~40 x 30 Saha recursions), so one thread runs
 // one cell in the reference's exact order of operations; the partition functions are the
@@ -252,6 +253,34 @@ __global__ __launch_bounds__(64) void k_eos_vjp(Dev d, const double *__restrict_
     for (int l = 0; l < 30; ++l) out_mf[(size_t)i * 30 + l] = (gx[l] - sxg) / H;
     out_mass[i] = gm;
     out_T[i] = gT;
+}
+
+// eos_update.f:47-58 with in_nlte: nlte_ndens(i,1:2,1), the H I and H II number densities
+// n_ii natom natomfr(1) / vol of the last solve (its last Saha evaluation, as the populations)
+__global__ __launch_bounds__(64) void k_eos_ndens(Dev d, const double *__restrict__ temp,
+                                                  const double *__restrict__ mass,
+                                                  const double *__restrict__ natom,
+                                                  const double *__restrict__ natomfr,
+                                                  double *__restrict__ ndens)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx) return;
+    double *o = ndens + 2 * (size_t)i;
+    const double s2 = d.eos_sahac2[i];
+    if (!(mass[i] > 0.0) || !(s2 > 0.0)) { o[0] = 0.0; o[1] = 0.0; return; }
+    double n[31];
+    for (int m = 0; m < 31; ++m) n[m] = 0.0;
+    saha_element(d, 1, 1.0 / (PC_KB * temp[i]), s2, n);
+    const double fr = natomfr[(size_t)i * 30];
+    o[0] = n[0] * natom[i] * fr / d.vol[i];
+    o[1] = n[1] * natom[i] * fr / d.vol[i];
+}
+
+void launch_eos_ndens(slgpu_ctx *c, const double *temp, const double *mass, const double *natom,
+                      const double *natomfr, double *ndens)
+{
+    hipLaunchKernelGGL(k_eos_ndens, dim3((c->nx + 63) / 64), dim3(64), 0, c->stream, c->d, temp,
+                       mass, natom, natomfr, ndens);
 }
 
 void launch_eos(slgpu_ctx *c, const double *temp, const double *mass, const double *massfr,

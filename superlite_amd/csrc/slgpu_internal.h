@@ -172,6 +172,8 @@ void launch_bfxs(slgpu_ctx *c, const float *d_ph1, const float *d_ph2, const int
 void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
                       const double *d_nelec, const double *d_oc, const double *d_ginv,
                       int flags);
+void launch_eos_ndens(slgpu_ctx *c, const double *temp, const double *mass, const double *natom,
+                      const double *natomfr, double *ndens);
 void launch_nlte_rad_est(slgpu_ctx *c, double *radtemp, double *jrad);
 void launch_opacity(slgpu_ctx *c, int flags);
 void launch_opacity_finish(slgpu_ctx *c);

@@ -123,6 +123,13 @@ module slgpu_iface
       type(c_ptr), value :: cap64, cap32, capemit, sig, capgrey, capemitgrey, capross, emiss, npop
     end function
 
+    ! nlte_ndens(i,1:2,1) of eos_update.f:52-57 from the last slgpu_eos (2*nc, H I then H II)
+    integer(c_int) function slgpu_eos_nlte_ndens(ctx, ndens) bind(C, name="slgpu_eos_nlte_ndens")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ndens(*)
+    end function
+
     ! nlte_rad_est (nltemod.f:231-260): call after slgpu_transport with grd_radtemp (in/out);
     ! jrad (ng*nc) may be c_null_ptr
     integer(c_int) function slgpu_nlte_rad_est(ctx, radtemp, jrad) bind(C, name="slgpu_nlte_rad_est")

@@ -30,7 +30,8 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_transport", "slgpu_fluxtally", "slgpu_get_particles", "slgpu_sample_groups",
            "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
            "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp",
-           "slgpu_set_nlte_data", "slgpu_opacity_build_nlte", "slgpu_nlte_rad_est")
+           "slgpu_set_nlte_data", "slgpu_opacity_build_nlte", "slgpu_nlte_rad_est",
+           "slgpu_eos_nlte_ndens")
 
 
 class SlgpuError(RuntimeError):
@@ -75,6 +76,7 @@ def lib():
         L.slgpu_set_nlte_data.argtypes = [_vp, _i, _i, _i] + [_vp] * 18
         L.slgpu_opacity_build_nlte.argtypes = [_vp, ctypes.POINTER(Gas), _vp, _vp, _i] + [_vp] * 9
         L.slgpu_nlte_rad_est.argtypes = [_vp, _vp, _vp]
+        L.slgpu_eos_nlte_ndens.argtypes = [_vp, _vp]
         L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
         L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
         L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _vp]
@@ -311,6 +313,12 @@ class Context:
         out["nelec"] = ne
         out["natomfr"] = out["natomfr"].T.copy()
         out["oc"], out["ginv"] = out["oc"][:, : self.nion], out["ginv"][:, : self.nion]
+        return out
+
+    def eos_nlte_ndens(self):
+        """nlte_ndens of the last eos(): (nc, 2) H I and H II number densities"""
+        out = np.zeros((self.nx, 2))
+        self._check(self.L.slgpu_eos_nlte_ndens(self.h, _p(out)))
         return out
 
     def eos_vjp(self, grad_natom=None, grad_natomfr=None, grad_nelec=None, grad_oc=None):

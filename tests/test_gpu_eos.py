@@ -59,6 +59,25 @@ def test_eos_forward_vs_oracle(S, tables, name):
     ctx.close()
 
 
+def test_eos_nlte_ndens_c2(S, tables):
+    """nlte_ndens (eos_update.f:52-57) of the device EOS at c2: the H I density equals the
+    H I ground-level population times q/g (the same Saha evaluation), and the two stages add
+    up to the hydrogen density natom natomfr(1)/vol"""
+    say("case: device EOS nlte_ndens c2")
+    ctx, w, st, wl, lines = _ctx(S, "c2", tables)
+    g = ctx.eos(st["temp"], st["mass"], st["massfr"])
+    nd = ctx.eos_nlte_ndens()
+    ed = synth.load_eos_data("c2")
+    kH = [k for k in range(len(st["ion_iz"])) if st["ion_iz"][k] == 1 and st["ion_ii"][k] == 1][0]
+    m = st["mass"] > 0
+    n1 = g["oc"][:, kH] * ed["q"][0][0] / ed["g1"][0][0]
+    np.testing.assert_allclose(nd[m, 0], n1[m], rtol=1e-14, atol=0)
+    nh = g["natom"] * g["natomfr"][0] / st["vol"]
+    np.testing.assert_allclose(nd[m].sum(axis=1), nh[m], rtol=1e-14, atol=0)
+    assert np.all(nd[m] >= 0) and np.all(nd[~m] == 0)
+    ctx.close()
+
+
 @pytest.mark.parametrize("name", ["c1", "c5"])
 def test_eos_second_call_vs_reference(S, tables, golden, name):
     say(f"case: device EOS second call {name}")

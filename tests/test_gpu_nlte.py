@@ -205,3 +205,35 @@ def test_nlte_call_order(c2):
     with pytest.raises(S.SlgpuError):
         ctx.ddmc_prep(radtemp, w.lbol)
     ctx.close()
+
+
+def test_nlte_iteration_device_chain_c2(c2):
+    """one NLTE iteration with every step on the device, in both precision forms: the EOS and
+    its nlte_ndens, the NLTE opacity, the DDMC prep, sources, transport, flux tally and
+    nlte_rad_est (superlite.f90:157-208 with in_nlte); the densities equal those derived from
+    the reference's own EOS state, and the iteration conserves energy"""
+    from superlite_amd import slgpu as S
+    w, st, wl, lines, ndens, radtemp = c2
+    sp = synth.load_nlte_data()
+    for prec in (S.SLGPU_FP64, S.SLGPU_MIXED):
+        say(f"case: nlte iteration on the device c2, precision {prec}")
+        ctx = S.setup_workload(w, st, wl, lines, synth.load_tables(), precision=prec,
+                               npartmax=1 << 16, seed=3)
+        ctx.set_eos_data(synth.load_eos_data("c2"))
+        ctx.set_nlte_data(sp)
+        e = ctx.eos(st["temp"], st["mass"], st["massfr"])
+        nd = ctx.eos_nlte_ndens()
+        nh = e["natom"] * e["natomfr"][0] / st["vol"]
+        np.testing.assert_allclose(nd[:, 0], ndens[:, 0], rtol=1e-12)
+        assert np.all(np.abs(nd[:, 1] - ndens[:, 1]) <= 1e-13 * nh)
+        s2 = dict(st)
+        s2.update(natom=e["natom"], natomfr=e["natomfr"], nelec=e["nelec"], oc=e["oc"], ginv=e["ginv"])
+        r = S.iteration_nlte(ctx, s2, 1 << 16, 1, w.lbol, radtemp, nd)
+        ctx.close()
+        tr = r["transport"]
+        cons = abs(tr["estart"] - (tr["eout"] + tr["evelo"] + tr["edead"])) / tr["estart"]
+        say(f"steps {tr['nstepimc']} {tr['nstepddmc']} flux {tr['nflux']} conservation {cons:.1e}")
+        assert cons < 1e-12 and tr["nerr"] == 0
+        assert r["flux"]["lumnum"].sum() == tr["nflux"]
+        assert np.all(np.isfinite(r["radtemp"])) and np.all(r["radtemp"] > 0)
+        np.testing.assert_allclose(r["opacity"]["npop"].sum(axis=1)[st["mass"] > 0], 1.0, rtol=1e-10)
