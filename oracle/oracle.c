@@ -1893,3 +1893,14 @@ or_u32x4 or_philox_block(or_u32x4 c, uint32_t k0, uint32_t k1)
 {
     return or_philox4x32_10(c, k0, k1);
 }
+
+/* exported stream draws, for the tests of the stream layout: n draws of the sequential
+ * stream (step = 0) or of transport step `step` (step >= 1) of particle pid */
+void or_stream_draws(uint32_t seed, uint32_t iter, uint32_t pid, uint32_t phase,
+                     uint32_t step, int n, double *out)
+{
+    or_stream s;
+    or_stream_init(&s, seed, iter, pid, phase);
+    if (step > 0) or_stream_step(&s, step);
+    for (int k = 0; k < n; ++k) out[k] = or_draw(&s);
+}

@@ -184,6 +184,11 @@ int64_t or_sourcenumbers(int nc, const double *emit, int64_t nstot, int *nvol);
  *      key (seed, iteration), counter (particle, block, phase 0=source 1=transport) ---- */
 int or_iteration(const or_model *m, uint32_t seed, uint32_t iteration, or_result *res);
 
+/* n draws of a particle's random stream: the sequential stream (step = 0; sources) or that
+ * of transport step `step` >= 1 */
+void or_stream_draws(uint32_t seed, uint32_t iter, uint32_t pid, uint32_t phase,
+                     uint32_t step, int n, double *out);
+
 /* source sampling only (interior_source.f90:57-168 + source_transformdirection.f90:23-46) */
 int or_sources(const or_model *m, uint32_t seed, uint32_t iteration,
                double *x, double *mu, double *om, double *e, double *e0, double *wl);

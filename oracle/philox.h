@@ -42,9 +42,17 @@ static inline double or_u01(uint32_t a, uint32_t b)
     return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+/* 42-bit uniform strictly inside (0,1): (hi * 2^10 + lo + 0.5) * 2^-42. */
+static inline double or_u42(uint32_t hi, uint32_t lo)
+{
+    uint64_t m = ((uint64_t)hi << 10) | (uint64_t)(lo & 0x3FFu);
+    return ((double)m + 0.5) * (1.0 / 4398046511104.0);
+}
+
 /* Per-particle stream: key = (seed, iteration).  Sequential mode (sources): draw k uses
- * counter (particle, k/2, phase, 0).  Stepped mode (transport): draw j of step s uses
- * counter (particle, s, phase, j/2).  Words (0,1) serve even draws, (2,3) odd ones. */
+ * counter (particle, k/2, phase, 0); words (0,1) serve even draws, (2,3) odd ones.  Stepped
+ * mode (transport): draw j of step s uses counter (particle, s, phase, j/3), three 42-bit
+ * uniforms per block: word j%3 above the 10 bits of word 3 starting at bit 10(j%3). */
 typedef struct {
     uint32_t k0, k1, pid, phase;
     uint32_t ndraw, step;
@@ -68,9 +76,16 @@ static inline void or_stream_step(or_stream *s, uint32_t step)
 static inline double or_draw(or_stream *s)
 {
     uint32_t k = s->ndraw++;
+    if (s->stepped) {
+        uint32_t j = k % 3u;
+        if (j == 0) {
+            or_u32x4 c = {{s->pid, s->step, s->phase, k / 3u}};
+            s->buf = or_philox4x32_10(c, s->k0, s->k1);
+        }
+        return or_u42(s->buf.v[j], s->buf.v[3] >> (10u * j));
+    }
     if ((k & 1u) == 0) {
         or_u32x4 c = {{s->pid, k >> 1, s->phase, 0u}};
-        if (s->stepped) { c.v[1] = s->step; c.v[3] = k >> 1; }
         s->buf = or_philox4x32_10(c, s->k0, s->k1);
         return or_u01(s->buf.v[0], s->buf.v[1]);
     }

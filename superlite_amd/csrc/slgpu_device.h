@@ -125,20 +125,23 @@ __device__ __forceinline__ double planck(double wl, double temp)
 
 // ---------------------------------------------------------------------------
 // Counter-based RNG: Philox-4x32-10 (Salmon et al., SC'11).  One stream per particle, key
-// (seed, iteration).  Sequential mode (sources): draw k uses counter (particle, k/2, phase,
-// 0).  Stepped mode (transport): draw j of step s uses counter (particle, s, phase, j/2),
-// and begin_step computes the step's first block at the top of the step, where every lane
-// of the pass is active, instead of wherever a lane's draw count happens to be even.
-// Words (0,1) serve even draws and (2,3) odd ones; uniforms lie in (0,1].
+// (seed, iteration); uniforms lie strictly inside (0,1).  (The reference's rnd_r has 32
+// bits of resolution, randommod.f:73-92.)
+//   Philox (sources): draw k uses counter (particle, k/2, phase, 0); words (0,1) serve even
+//   draws and (2,3) odd ones, 53-bit uniforms.
+//   StepRng (transport): draw j of step s uses counter (particle, s, phase, j/3); a block's
+//   three draws are 42-bit uniforms, word j%3 above the 10 bits of word 3 from bit
+//   10(j%3).  One block covers a DDMC step's three draws, and an IMC step's two distances
+//   and first event draw.  begin_step computes the step's first block at the top of the
+//   step, where every lane of the pass is active.
 struct Philox {
-    uint32_t k0, k1, pid, phase, ndraw, step;
-    uint32_t w0, w1, w2, w3;
-    bool stepped;
+    uint32_t k0, k1, pid, phase, ndraw;
+    uint32_t w2, w3;
 
     __device__ __forceinline__ void init(uint32_t seed, uint32_t iter, uint32_t p, uint32_t ph)
     {
-        k0 = seed; k1 = iter; pid = p; phase = ph; ndraw = 0; step = 0; stepped = false;
-        w0 = w1 = w2 = w3 = 0;
+        k0 = seed; k1 = iter; pid = p; phase = ph; ndraw = 0;
+        w2 = w3 = 0;
     }
 
     __device__ __forceinline__ static void block(uint32_t c0, uint32_t c1, uint32_t c2,
@@ -168,29 +171,61 @@ struct Philox {
         return fma((double)(a >> 5), 67108864.0, (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
     }
 
-    __device__ __forceinline__ void fill(uint32_t j)
+    // (m + 0.5) * 2^-42 for the 42-bit m = hi * 2^10 + lo (exact in FP64)
+    __device__ __forceinline__ static double u42(uint32_t hi, uint32_t lo)
     {
-        uint32_t o[4];
-        if (stepped) block(pid, step, phase, j, k0, k1, o);
-        else block(pid, j, phase, 0u, k0, k1, o);
-        w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
-    }
-
-    // start transport step s with its first block computed
-    __device__ __forceinline__ void begin_step(uint32_t s)
-    {
-        stepped = true; step = s; ndraw = 0;
-        fill(0u);
+        return fma((double)hi, 1024.0, (double)lo + 0.5) * (1.0 / 4398046511104.0);
     }
 
     __device__ __forceinline__ double draw()
     {
         const uint32_t k = ndraw++;
         if ((k & 1u) == 0) {
-            if (!(stepped && k == 0)) fill(k >> 1);
-            return u01(w0, w1);
+            uint32_t o[4];
+            block(pid, k >> 1, phase, 0u, k0, k1, o);
+            w2 = o[2]; w3 = o[3];
+            return u01(o[0], o[1]);
         }
         return u01(w2, w3);
+    }
+};
+
+struct StepRng {
+    static constexpr uint32_t PHASE = 1;    // the transport's counter phase (sources: 0)
+    uint32_t k0, k1;        // key (seed, iteration): set once, the same in every lane
+    uint32_t pid, step;
+    uint32_t cnt;           // 4 * (block index) + draws taken from the current block
+    uint32_t a, b, c, lo;   // a serves the next draw; lo holds its 10 low bits at the bottom
+
+    __device__ __forceinline__ void key(uint32_t seed, uint32_t iter) { k0 = seed; k1 = iter; }
+
+    // a new particle's stream
+    __device__ __forceinline__ void init(uint32_t p)
+    {
+        pid = p; step = 0; cnt = 0;
+        a = b = c = lo = 0;
+    }
+
+    __device__ __forceinline__ void fill(uint32_t j)
+    {
+        uint32_t o[4];
+        Philox::block(pid, step, PHASE, j, k0, k1, o);
+        a = o[0]; b = o[1]; c = o[2]; lo = o[3];
+    }
+
+    // start transport step s with its first block computed
+    __device__ __forceinline__ void begin_step(uint32_t s)
+    {
+        step = s; cnt = 0;
+        fill(0u);
+    }
+
+    __device__ __forceinline__ double draw()
+    {
+        if ((cnt & 3u) == 3u) { cnt += 1u; fill(cnt >> 2); }
+        const double u = Philox::u42(a, lo & 0x3FFu);
+        a = b; b = c; lo >>= 10; ++cnt;
+        return u;
     }
 };
 

@@ -185,7 +185,7 @@ __device__ __forceinline__ double thomson_aniso_mu(double r1)
 }
 
 template <bool M>
-__device__ int transport11(const Dev &d, Ptcl &p, Philox &s, double &erd, double &jrd,
+__device__ int transport11(const Dev &d, Ptcl &p, StepRng &s, double &erd, double &jrd,
                            double &eamp, double &evelo)
 {
     const int ng = d.ng, nx = d.nx;
@@ -513,7 +513,7 @@ void launch_ddmc_probs(slgpu_ctx *c)
 enum { EV_DOP = 0, EV_LEFT = 1, EV_RIGHT = 2, EV_SCAT = 3 };
 
 template <bool M>
-__device__ int diffusion11(const Dev &d, Ptcl &p, Philox &s, double &erd, double &jrd,
+__device__ int diffusion11(const Dev &d, Ptcl &p, StepRng &s, double &erd, double &jrd,
                            double &evelo)
 {
     const int ng = d.ng, nx = d.nx;
@@ -746,7 +746,8 @@ void k_transport(Dev d, uint32_t iteration)
     double run_e = 0.0, run_j = 0.0;
     double *jrad = d.jrad + (size_t)(blockIdx.x % d.jrad_copies) * nx * ng;
     Ptcl p;
-    Philox s;
+    StepRng s;
+    s.key(d.seed, iteration);
     int64_t ip = -1, istep = 0;
     bool have = false;
     for (;;) {
@@ -778,7 +779,7 @@ void k_transport(Dev d, uint32_t iteration)
             p.stat = 0;
             p.idist = 0;
             p.nflag = 0;
-            s.init(d.seed, iteration, (uint32_t)ip, 1u);
+            s.init((uint32_t)ip);
             istep = 0;
             have = true;
         }

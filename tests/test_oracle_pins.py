@@ -35,6 +35,53 @@ def test_philox_kat(ctr, key, expect):
     assert tuple(out.v) == expect
 
 
+def test_stream_layout():
+    """The particle streams: a transport step's draw j is the 42-bit uniform of block
+    (particle, step, 1, j // 3), word j % 3 above bits 10 (j % 3).. of word 3; a sequential
+    (source) draw k the 53-bit uniform of block (particle, k // 2, 0, 0), words (0, 1) or
+    (2, 3).  Checked against the blocks themselves, the resolution grid, and uniformity."""
+    import ctypes
+
+    from scipy import stats
+
+    class U4(ctypes.Structure):
+        _fields_ = [("v", ctypes.c_uint32 * 4)]
+    L = po.lib()
+    blk = L.or_philox_block
+    blk.restype = U4
+    blk.argtypes = [U4, ctypes.c_uint32, ctypes.c_uint32]
+    L.or_stream_draws.argtypes = [ctypes.c_uint32] * 5 + [ctypes.c_int, ctypes.c_void_p]
+
+    def block(c, k):
+        return blk(U4((ctypes.c_uint32 * 4)(*c)), *k).v
+
+    def draws(pid, phase, step, n, seed=9, it=4):
+        out = np.zeros(n)
+        L.or_stream_draws(seed, it, pid, phase, step, n, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+    for pid, step in ((0, 1), (12345, 7), (2 ** 24 - 1, 300000)):
+        d = draws(pid, 1, step, 10)
+        for j in range(10):
+            w = block((pid, step, 1, j // 3), (9, 4))
+            m = (int(w[j % 3]) << 10) | ((int(w[3]) >> (10 * (j % 3))) & 0x3FF)
+            assert d[j] == (m + 0.5) * 2.0 ** -42, (pid, step, j)
+    d = draws(77, 0, 0, 6)
+    for k in range(6):
+        w = block((77, k // 2, 0, 0), (9, 4))
+        a, b = (w[0], w[1]) if k % 2 == 0 else (w[2], w[3])
+        m = ((int(a) >> 5) << 26) | (int(b) >> 6)
+        assert d[k] == (m + 0.5) * 2.0 ** -53
+    # 42-bit grid, strictly inside (0,1), uniform (6 draws of 20000 steps)
+    u = np.concatenate([draws(5, 1, s, 6) for s in range(1, 20001)])
+    g = u * 2.0 ** 42 - 0.5
+    assert np.all(g == np.floor(g)) and np.all((u > 0) & (u < 1))
+    assert stats.kstest(u, "uniform").pvalue > 1e-4
+    # the three draws of one block are independent: their correlations vanish
+    t = u.reshape(-1, 6)[:, :3]
+    c = np.corrcoef(t.T)
+    assert np.max(np.abs(c - np.eye(3))) < 0.03
+
+
 # --- binsrch (MISC/binsrch.f90) -----------------------------------------------------
 def test_binsrch_golden(golden):
     g = golden("ref_c1_opacity.npz")
