@@ -16,9 +16,9 @@
 // on the schedule, and every lane of a pass computes its step's first block together.
 //
 // Tallies: the per-cell energy tally lives in LDS (nc doubles) per block and is flushed
-// with FP64 atomics at the end; consecutive events of one particle in the same
-// (cell, group) are merged in registers before the FP64 atomic into one of the grd_jrad
-// replicas (block-indexed), which k_jrad_reduce folds together afterwards.
+// with FP64 atomics at the end; a particle's consecutive events in one cell are merged in
+// registers before the LDS atomic.  Each event adds to grd_jrad with one FP64 atomic into
+// one of its replicas (block-indexed), which k_jrad_reduce folds together afterwards.
 #include "slgpu_device.h"
 #include "slgpu_internal.h"
 #include "slgpu_sample.h"
@@ -112,6 +112,13 @@ struct Ptcl {
     uint32_t nflag;
 };
 constexpr uint32_t NF_VALID = 1u;
+// A DDMC -> IMC conversion (diffusion11.f90:329-355, 402-423, 515-538, 587-618) is recorded
+// in nflag and completed at the particle's next pass, an IMC pass, where most lanes are
+// such conversions: bits 1-3 the kind, bit 4 set when the step's draw 2 was still unused.
+// The adiabatic factor of the converting step and that draw wait in the block's LDS.
+enum { PEND_SCAT = 1, PEND_LEFT = 2, PEND_RIGHT = 3, PEND_DOP = 4 };
+constexpr uint32_t NF_PEND = 7u << 1;
+constexpr uint32_t NF_R2FREE = 1u << 4;
 
 // TRANSPORT1/transport11.f90:86-489 (trn_noampfact = .true.)
 // Arithmetic of the precision form.  SLGPU_MIXED evaluates the sampled path lengths and
@@ -514,7 +521,7 @@ enum { EV_DOP = 0, EV_LEFT = 1, EV_RIGHT = 2, EV_SCAT = 3 };
 
 template <bool M>
 __device__ int diffusion11(const Dev &d, Ptcl &p, StepRng &s, double &erd, double &jrd,
-                           double &evelo)
+                           double &evelo, double &pend_f, double &pend_r2)
 {
     const int ng = d.ng, nx = d.nx;
     int ix = p.ix, ig = p.ig;
@@ -615,37 +622,15 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, StepRng &s, double &erd, doubl
                 lc = ix + 1;
             }
             if (fl < 0) fl = (int)group_flags(d, ig2, lc);
-            if (fl & (int)BT_THIN) {
-                // conversion to IMC (:329-355, 402-423, 515-538, 587-618)
-                p.itype = 1;
-                const bool face = (ev == EV_LEFT || ev == EV_RIGHT);
-                if (ev == EV_SCAT) {
-                    r1 = next();
-                    wl = 1.0 / ((1.0 - r1) * WLI(ig2) + r1 * WLI(ig2 + 1));
-                } else if (face) {
-                    r1 = next();
-                    wl = 1.0 / (r1 * WLI(iiig + 1) + (1.0 - r1) * WLI(iiig));
-                }
-                const double ra = next(), rb = next();
-                if (face) {
-                    const int xi = ev == EV_LEFT ? ix : ix + 1;
-                    x = XA(xi);
-                    vx = VXA(xi);
-                    mu = ra > rb ? ra : rb;
-                    if (ev == EV_LEFT) mu = -mu;
-                } else {
-                    mu = 1.0 - 2.0 * ra;
-                    x = shell_position<M>(d, ix, rb);
-                    vx = (VXA(ix) * (XA(ix + 1) - x) + VXA(ix + 1) * (x - XA(ix))) / DXX(ix);
-                }
-                mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
-                const double help = 1.0 / (1.0 - vx * mu * CINV);
-                evelo = evelo + e * (1.0 - help);
-                e = e * help;
-                e0 = e0 * help;
-                wl = wl * (1.0 - vx * mu * CINV);
-            }
             nflag = NF_VALID | ((uint32_t)fl & BT_LUMP);
+            if (fl & (int)BT_THIN) {
+                // conversion to IMC: recorded here, sampled by imc_entry at the next pass
+                p.itype = 1;
+                const uint32_t kind = ev == EV_SCAT ? PEND_SCAT
+                                    : (ev == EV_LEFT ? PEND_LEFT : (ev == EV_RIGHT ? PEND_RIGHT : PEND_DOP));
+                nflag = (kind << 1) | (r2_free ? NF_R2FREE : 0u) | ((uint32_t)fl & BT_LUMP);
+                if (r2_free) pend_r2 = r2;
+            }
             ig = ig2;
             if (ev == EV_LEFT) ix = ix - 1;
             else if (ev == EV_RIGHT) ix = ix + 1;
@@ -653,7 +638,11 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, StepRng &s, double &erd, doubl
     }
     p.nflag = nflag;
     if (adiabatic) {
-        if (x < XA(1)) {
+        if (nflag & NF_PEND) {
+            // after the conversion's frame change, at the next pass (its position lies in
+            // the shell, so the inner-boundary test below never fires for it)
+            pend_f = p_adiabatic<M>(DIVV(ix), ddmct);
+        } else if (x < XA(1)) {
             p.stat = 2;                                        // :623-626
         } else {
             const double f = p_adiabatic<M>(DIVV(ix), ddmct);
@@ -665,6 +654,60 @@ __device__ int diffusion11(const Dev &d, Ptcl &p, StepRng &s, double &erd, doubl
     p.x = x; p.mu = mu; p.e = e; p.e0 = e0; p.wl = wl; p.vx = vx;
     p.ix = ix; p.ig = ig;
     return ret;
+}
+
+// The recorded DDMC -> IMC conversion of step `step` (diffusion11.f90:329-355, 402-423,
+// 515-538, 587-618): the new wavelength, direction and position, the frame change, then the
+// step's adiabatic factor f, in the reference's order.  Its draws are that step's: draw 2 when
+// it was still unused (r2), then draws 3.. (block 1 of the step).
+template <bool M>
+__device__ void imc_entry(const Dev &d, Ptcl &p, const StepRng &s, uint32_t step, double &evelo,
+                          double f, double r2)
+{
+    const uint32_t kind = (p.nflag & NF_PEND) >> 1;
+    bool r2_free = (p.nflag & NF_R2FREE) != 0;
+    uint32_t o[4];
+    Philox::block(s.pid, step, StepRng::PHASE, 1u, s.k0, s.k1, o);
+    uint32_t j = 0;
+    auto next = [&]() -> double {
+        if (r2_free) { r2_free = false; return r2; }
+        const double u = Philox::u42(j == 0 ? o[0] : (j == 1 ? o[1] : o[2]), (o[3] >> (10u * j)) & 0x3FFu);
+        ++j;
+        return u;
+    };
+    const int ix = p.ix, ig = p.ig;
+    double x, mu, vx, e = p.e, e0 = p.e0, wl = p.wl;
+    const bool face = kind == PEND_LEFT || kind == PEND_RIGHT;
+    if (kind == PEND_SCAT) {
+        const double r1 = next();
+        wl = 1.0 / ((1.0 - r1) * WLI(ig) + r1 * WLI(ig + 1));
+    } else if (face) {
+        const double r1 = next();
+        wl = 1.0 / (r1 * WLI(ig + 1) + (1.0 - r1) * WLI(ig));
+    }
+    const double ra = next(), rb = next();
+    if (face) {
+        const int xi = kind == PEND_LEFT ? ix + 1 : ix;      // the face crossed into cell ix
+        x = XA(xi);
+        vx = VXA(xi);
+        mu = ra > rb ? ra : rb;
+        if (kind == PEND_LEFT) mu = -mu;
+    } else {
+        mu = 1.0 - 2.0 * ra;
+        x = shell_position<M>(d, ix, rb);
+        vx = (VXA(ix) * (XA(ix + 1) - x) + VXA(ix + 1) * (x - XA(ix))) / DXX(ix);
+    }
+    mu = (mu + vx * CINV) / (1.0 + vx * mu * CINV);
+    const double help = 1.0 / (1.0 - vx * mu * CINV);
+    evelo = evelo + e * (1.0 - help);
+    e = e * help;
+    e0 = e0 * help;
+    wl = wl * (1.0 - vx * mu * CINV);
+    evelo = evelo + e * (1.0 - f);
+    e = e * f;
+    e0 = e0 * f;
+    p.x = x; p.mu = mu; p.vx = vx; p.e = e; p.e0 = e0; p.wl = wl;
+    p.nflag = NF_VALID | (p.nflag & BT_LUMP);
 }
 
 // Block totals in LDS, indexed like Dev::tot and Dev::ct.  Whatever changes once per
@@ -736,14 +779,15 @@ void k_transport(Dev d, uint32_t iteration)
     extern __shared__ double s_tally[];
     const int ng = d.ng, nx = d.nx;
     __shared__ BlockTotals bt;
+    __shared__ double s_pend_f[256], s_pend_r2[256];       // a lane's recorded conversion
     for (int k = threadIdx.x; k < nx; k += blockDim.x) s_tally[k] = 0.0;
     if (threadIdx.x < 7) bt.tot[threadIdx.x] = 0.0;
     if (threadIdx.x < 9) bt.ct[threadIdx.x] = 0;
     __syncthreads();
     double evelo = 0.0;
     uint32_t nddmc = 0, nswap = 0;
-    int run_ic = -1, run_ig = -1;
-    double run_e = 0.0, run_j = 0.0;
+    int run_ic = -1;
+    double run_e = 0.0;
     double *jrad = d.jrad + (size_t)(blockIdx.x % d.jrad_copies) * nx * ng;
     Ptcl p;
     StepRng s;
@@ -789,6 +833,8 @@ void k_transport(Dev d, uint32_t iteration)
         const int n_imc = __popcll(__ballot(imc)), n_ddmc = __popcll(__ballot(!imc));
         if (imc != (n_imc * SLGPU_VOTE_IMC >= n_ddmc * SLGPU_VOTE_DDMC)) continue;
         istep++;
+        if (imc && (p.nflag & NF_PEND))
+            imc_entry<M>(d, p, s, (uint32_t)(istep - 1), evelo, s_pend_f[threadIdx.x], s_pend_r2[threadIdx.x]);
         s.begin_step((uint32_t)istep);
         const int icold = p.ix, igold = p.ig;
         double erd, jrd;
@@ -803,19 +849,19 @@ void k_transport(Dev d, uint32_t iteration)
             if (p.itype != 1) nswap++;
         } else {
             nddmc++;
-            ierr = diffusion11<M>(d, p, s, erd, jrd, evelo);
+            ierr = diffusion11<M>(d, p, s, erd, jrd, evelo, s_pend_f[threadIdx.x], s_pend_r2[threadIdx.x]);
             if (p.itype == 1) nswap++;
         }
-        if (icold != run_ic || igold != run_ig) {
-            if (run_ic > 0) {
-                atomicAdd(&s_tally[run_ic - 1], run_e);
-                atomicAdd(&jrad[(size_t)(run_ic - 1) * ng + run_ig - 1], run_j);
-            }
-            run_ic = icold; run_ig = igold; run_e = 0.0; run_j = 0.0;
+        // grd_jrad: one FP64 atomic per event into the block's replica (the group changes on
+        // almost every step, so merging a particle's events would save no atomics); the
+        // energy tally merges a particle's consecutive events in one cell before its LDS atomic
+        if (jrd != 0.0) atomicAdd(&jrad[(size_t)(icold - 1) * ng + igold - 1], jrd);
+        if (icold != run_ic) {
+            if (run_ic > 0) atomicAdd(&s_tally[run_ic - 1], run_e);
+            run_ic = icold; run_e = 0.0;
         }
         run_e += erd;
-        run_j += jrd;
-        if (p.itype == 1 && p.stat == 0) {                     // particle_advance.f90:234-250
+        if (p.itype == 1 && p.stat == 0 && !(p.nflag & NF_PEND)) {   // particle_advance.f90:234-250
             if (p.x > XA(p.ix + 1) || p.x < XA(p.ix) || p.x != p.x) {
                 if (ierr == 0) ierr = -99;
             }
@@ -824,6 +870,8 @@ void k_transport(Dev d, uint32_t iteration)
         if (ierr == 0 && p.stat == 0 && istep >= d.max_steps) ierr = 200;
         if (ierr > 0) { atomicAdd(&bt.ct[7], 1ull); p.stat = 2; }
         if (p.stat != 0) {
+            if (p.nflag & NF_PEND)                             // stopped by max_steps right after
+                imc_entry<M>(d, p, s, (uint32_t)istep, evelo, s_pend_f[threadIdx.x], s_pend_r2[threadIdx.x]);
             atomicMax(&bt.ct[3], (unsigned long long)istep);
             atomicAdd(&bt.ct[1], (unsigned long long)istep);
             if (p.stat == 2) { atomicAdd(&bt.ct[6], 1ull); atomicAdd(&bt.tot[4], p.e); }
@@ -836,10 +884,7 @@ void k_transport(Dev d, uint32_t iteration)
             have = false;
         }
     }
-    if (run_ic > 0) {
-        atomicAdd(&s_tally[run_ic - 1], run_e);
-        atomicAdd(&jrad[(size_t)(run_ic - 1) * ng + run_ig - 1], run_j);
-    }
+    if (run_ic > 0) atomicAdd(&s_tally[run_ic - 1], run_e);
     __syncthreads();
     for (int k = threadIdx.x; k < nx; k += blockDim.x)
         if (s_tally[k] != 0.0) atomicAdd(&d.tally[k], s_tally[k]);
