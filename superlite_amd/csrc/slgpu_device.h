@@ -148,14 +148,16 @@ struct Philox {
                                                  uint32_t c3, uint32_t k0, uint32_t k1,
                                                  uint32_t out[4])
     {
-        // rolled: one copy of the round per call site keeps the transport kernel's code
-        // small enough for the instruction cache.  Each 32x32 product is formed as one
-        // 64-bit product, which compiles to one v_mad_u64_u32 instead of a v_mul_lo_u32 and
-        // a v_mul_hi_u32; same bits.
-        #pragma unroll 1
+        // Two rounds per loop trip: the rotation of (c0..c3) then costs no register moves,
+        // and the body stays small for the instruction cache.  Each 32x32 product is formed
+        // as one 64-bit product (one v_mad_u64_u32 instead of v_mul_lo_u32 + v_mul_hi_u32;
+        // same bits), and each three-input XOR is one v_bitop3_b32 (truth table 0x96 =
+        // a ^ b ^ c): four VALU instructions per round.
+        #pragma unroll 2
         for (int r = 0; r < 10; ++r) {
             const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
-            const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+            const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+            const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
             c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
