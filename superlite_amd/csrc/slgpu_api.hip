@@ -459,15 +459,26 @@ static int upload_lines(slgpu_ctx *c)
     std::vector<int> gstart(ng + 2, 0);
     for (int q = 1; q <= ng; ++q) gstart[q + 1] = gstart[q] + cnt[q];
     c->nline_used = (int64_t)recs.size();
+    // the backward's copy: each group's lines ordered by ion (stable), so a run of one ion's
+    // lines shares its population and its gradient accumulator
+    std::vector<LineRec> recs_ion(recs);
+    for (int q = 1; q <= ng; ++q)
+        std::stable_sort(recs_ion.begin() + gstart[q], recs_ion.begin() + gstart[q + 1],
+                         [](const LineRec &a, const LineRec &b) { return a.ion < b.ion; });
     dfree(c, (void *)c->d.lines);
+    dfree(c, (void *)c->d.lines_ion);
     dfree(c, (void *)c->d.gstart);
     int rc = 0;
     rc |= dalloc(c, (LineRec **)&c->d.lines, recs.size());
+    rc |= dalloc(c, (LineRec **)&c->d.lines_ion, recs.size());
     rc |= dalloc(c, (int **)&c->d.gstart, ng + 2);
     if (rc) return rc;
-    if (!recs.empty())
+    if (!recs.empty()) {
         HIPCHK(hipMemcpyAsync((void *)c->d.lines, recs.data(), recs.size() * sizeof(LineRec),
                               hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync((void *)c->d.lines_ion, recs_ion.data(), recs_ion.size() * sizeof(LineRec),
+                              hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(hipMemcpyAsync((void *)c->d.gstart, gstart.data(), (ng + 2) * sizeof(int),
                           hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -49,6 +49,7 @@ struct Dev {
     const double *wl, *wlinv;
     // lines
     const LineRec *lines;
+    const LineRec *lines_ion;   // the same records, sorted by ion within each group (backward)
     const int *gstart;          // ng+2: lines of group g (1-based) are [gstart[g], gstart[g+1])
     // tables
     double *bfxs;               // ng*nion, xs(ig, k)*1e-18 (Mb -> cm^2)

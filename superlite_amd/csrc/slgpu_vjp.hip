@@ -8,7 +8,12 @@
 //                   their direct T and sigma_Thomson terms
 //   k_vjp_opacity   one thread per (cell, group), 64 cells x 16 groups per block: the bb
 //                   line terms recomputed (not stored) and ff, with per-ion and
-//                   per-element accumulators in LDS flushed once per block
+//                   per-element accumulators in LDS flushed once per block; bb in
+//                   512-thread blocks and ff in a launch of its own when the staged
+//                   populations fit.  The bb loop
+//                   walks a copy of the line list ordered by ion within each group, so
+//                   each run of one ion's lines reads its population once and adds to its
+//                   accumulator once
 //   k_vjp_bf        the bf part, a contraction over groups of the cross sections with
 //                   dL/dcap
 //   k_vjp_finish    one thread per cell: chain rule through hckt = hc/kT, the ff prefactor
@@ -129,23 +134,29 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v)
 // of 64 line records.  A population that is not positive contributes nothing (the forward
 // skips it); it is staged as -1 in the precision form, so that a positive one that
 // underflows FP32 (+0) still contributes its derivative, and as 0 in FP64.  The bf part is k_vjp_bf.
-template <bool M, bool S>
-__global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ffc1,
-                                                     const double *__restrict__ rgg_tab)
+// P selects the parts: 1 the bb lines, 2 ff, 3 both; NWV waves per block share its 64 cells'
+// accumulators, each wave taking VJP_GROUPS / NWV of the block's groups.
+template <bool M, bool S, int P, int NWV>
+__global__ __launch_bounds__(NWV * 64)
+__attribute__((amdgpu_waves_per_eu(P == 1 ? 6 : 1, 8)))
+void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg_tab)
 {
     using PT = typename std::conditional<M, float, double>::type;
+    constexpr bool BB = (P & 1) != 0, FF = (P & 2) != 0;
+    constexpr int NT = NWV * 64;
     extern __shared__ double s_acc[];
-    __shared__ LineRec s_lines[4][64];
+    __shared__ LineRec s_lines[BB ? NWV : 1][64];
     const int nc = d.nx, ng = d.ng, nion = d.nion, nel = d.nel;
-    double *s_bb = s_acc, *s_fr = s_acc + (size_t)nion * 64;
-    PT *s_pop = reinterpret_cast<PT *>(s_acc + (size_t)(nion + nel) * 64);
-    for (int k = threadIdx.x; k < (nion + nel) * 64; k += 256) s_acc[k] = 0.0;
+    const int nrow_b = BB ? nion : 0, nrow_f = FF ? nel : 0;
+    double *s_bb = s_acc, *s_fr = s_acc + (size_t)nrow_b * 64;
+    PT *s_pop = reinterpret_cast<PT *>(s_acc + (size_t)(nrow_b + nrow_f) * 64);
+    for (int k = threadIdx.x; k < (nrow_b + nrow_f) * 64; k += NT) s_acc[k] = 0.0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     const bool act = i < nc;
     const int ic = act ? i : nc - 1;
-    if constexpr (S) {
-        for (int k = threadIdx.x; k < nion * 64; k += 256) {
+    if constexpr (S && BB) {
+        for (int k = threadIdx.x; k < nion * 64; k += NT) {
             const int cc = blockIdx.x * 64 + (k & 63);
             const double o = cc < nc ? d.ocg[(size_t)(k >> 6) * nc + cc] : 0.0;
             s_pop[k] = o > 0.0 ? (PT)o : (PT)(M ? -1 : 0);
@@ -154,12 +165,12 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
     __syncthreads();
     const double hckt = d.hckt[ic];
     double dh_acc = 0.0, dhl_acc = 0.0;
-    for (int q = 0; q < VJP_GROUPS / 4; ++q) {
-        const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * VJP_GROUPS + wave * (VJP_GROUPS / 4) + q) + 1;
+    for (int q = 0; q < VJP_GROUPS / NWV; ++q) {
+        const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * VJP_GROUPS + wave * (VJP_GROUPS / NWV) + q) + 1;
         if (g > ng) break;
         const double G = act ? d.gcapT[(size_t)(g - 1) * nc + i] : 0.0;
         const double wlo = d.wl[g - 1], whi = d.wl[g];
-        if (!(flags & SLGPU_NOBB)) {
+        if (BB && !(flags & SLGPU_NOBB)) {
             const double wlc = 0.5 * (wlo + whi);
             const double wlinv = 1.0 / wlc;
             const double dwl = whi - wlo;
@@ -173,29 +184,43 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
                 const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
                 float a0 = 0.0f, c0 = 0.0f, a1 = 0.0f, c1 = 0.0f;
                 if constexpr (S) {
+                    // the group's lines by ion: one population load and one accumulator add
+                    // per run of an ion's lines (the run test is wave-uniform)
                     LineRec *ln = s_lines[wave];
+                    int cur = -1;
+                    float ofc = -1.0f;
+                    double sb = 0.0;
                     for (int base0 = l0; base0 < l1; base0 += 64) {
                         const int n = l1 - base0 < 64 ? l1 - base0 : 64;
                         wave_lds_sync();
                         if (lane < n) {
-                            LineRec r = d.lines[base0 + lane];
+                            LineRec r = d.lines_ion[base0 + lane];
                             r.gxs = phi32 * r.gxs;             // base's first product
                             ln[lane] = r;
                         }
                         wave_lds_sync();
-                        #pragma unroll 2
+                        // the next record is read while this one is evaluated
+                        LineRec rn = ln[0];
                         for (int j = 0; j < n; ++j) {
-                            const LineRec r = ln[j];
-                            const float of = s_pop[r.ion * 64 + lane];
-                            if (of >= 0.0f) {
+                            const LineRec r = rn;
+                            rn = ln[j + 1 < n ? j + 1 : j];
+                            const int ion = __builtin_amdgcn_readfirstlane(r.ion);
+                            if (ion != cur) {
+                                if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
+                                cur = ion;
+                                sb = 0.0;
+                                ofc = s_pop[ion * 64 + lane];
+                            }
+                            if (ofc >= 0.0f) {
                                 const float base = r.gxs * exp_neg32(r.chilw * hk32) * ef32;
-                                const float term = base * of;
+                                const float term = base * ofc;
                                 kahan_add(a0, c0, term);
                                 kahan_add(a1, c1, term * r.chilw);
-                                atomicAdd(&s_bb[r.ion * 64 + lane], G * (double)base);
+                                sb += (double)base;
                             }
                         }
                     }
+                    if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
                 } else {
                     for (int l = l0; l < l1; ++l) {
                         const LineRec r = d.lines[l];
@@ -214,23 +239,34 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
             } else {
                 if constexpr (S) {
                     LineRec *ln = s_lines[wave];
+                    int cur = -1;
+                    double oc_ = 0.0, sb = 0.0;
                     for (int base0 = l0; base0 < l1; base0 += 64) {
                         const int n = l1 - base0 < 64 ? l1 - base0 : 64;
                         wave_lds_sync();
-                        if (lane < n) ln[lane] = d.lines[base0 + lane];
+                        if (lane < n) ln[lane] = d.lines_ion[base0 + lane];
                         wave_lds_sync();
+                        LineRec rn = ln[0];
                         for (int j = 0; j < n; ++j) {
-                            const LineRec r = ln[j];
-                            const double o = s_pop[r.ion * 64 + lane];
-                            if (o > 0.0) {
+                            const LineRec r = rn;
+                            rn = ln[j + 1 < n ? j + 1 : j];
+                            const int ion = __builtin_amdgcn_readfirstlane(r.ion);
+                            if (ion != cur) {
+                                if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
+                                cur = ion;
+                                sb = 0.0;
+                                oc_ = s_pop[ion * 64 + lane];
+                            }
+                            if (oc_ > 0.0) {
                                 const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
-                                const double term = base * o;
+                                const double term = base * oc_;
                                 s0 += term;
                                 s1 += term * (double)r.chilw;
-                                atomicAdd(&s_bb[r.ion * 64 + lane], G * base);
+                                sb += base;
                             }
                         }
                     }
+                    if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
                 } else {
                     for (int l = l0; l < l1; ++l) {
                         const LineRec r = d.lines[l];
@@ -247,7 +283,7 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
             }
             dh_acc += G * (-s1 + s0 * wlinv * ec / expfac);
         }
-        if (!(flags & SLGPU_NOFF)) {
+        if (FF && !(flags & SLGPU_NOFF)) {
             const double wl = wlo;
             const double u = hckt * (1.0 / wl);
             const double ru = 10.0 * (log10(u) + 4.0) + 1.0;
@@ -282,16 +318,16 @@ __global__ __launch_bounds__(256) void k_vjp_opacity(Dev d, int flags, double ff
         }
     }
     if (act) {
-        atomicAdd(&d.gcell[2 * nc + i], dh_acc);
-        atomicAdd(&d.gcell[3 * nc + i], dhl_acc);
+        if (dh_acc != 0.0) atomicAdd(&d.gcell[2 * nc + i], dh_acc);
+        if (FF) atomicAdd(&d.gcell[3 * nc + i], dhl_acc);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < (nion + nel) * 64; k += 256) {
+    for (int k = threadIdx.x; k < (nrow_b + nrow_f) * 64; k += NT) {
         const int row = k >> 6, cell = blockIdx.x * 64 + (k & 63);
         const double v = s_acc[k];
         if (cell < nc && v != 0.0) {
-            if (row < nion) atomicAdd(&d.goc[(size_t)row * nc + cell], v * d.ginvT[(size_t)row * nc + cell]);
-            else atomicAdd(&d.gfr[(size_t)(row - nion) * nc + cell], v);
+            if (row < nrow_b) atomicAdd(&d.goc[(size_t)row * nc + cell], v * d.ginvT[(size_t)row * nc + cell]);
+            else atomicAdd(&d.gfr[(size_t)(row - nrow_b) * nc + cell], v);
         }
     }
 }
@@ -359,19 +395,35 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     hipLaunchKernelGGL(k_vjp_means, dim3(nc), dim3(256), 0, c->stream, c->d, gc, gp, gr);
     const bool mixed = c->d.precision == SLGPU_MIXED;
     const size_t acc_b = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
-    const size_t pop_b = (flags & SLGPU_NOBB) ? 0 : (size_t)c->nion * 64 * (mixed ? sizeof(float) : sizeof(double));
-    // staged populations while the block's LDS stays within 64 KiB (4 KiB of line slots)
-    const bool staged = acc_b + pop_b + 4096 <= 65536;
-    const size_t lds = acc_b + (staged ? pop_b : 0);
+    const size_t bb_b = (size_t)c->nion * 64 * sizeof(double), ff_b = (size_t)c->d.nel * 64 * sizeof(double);
+    const size_t pop_b = (size_t)c->nion * 64 * (mixed ? sizeof(float) : sizeof(double));
+    // With the populations staged and the line slots of 8 waves within 64 KiB of LDS, the bb
+    // lines run in 512-thread blocks (8 waves x 2 groups sharing the cells' accumulators) and
+    // ff in a kernel of its own; otherwise both in one 256-thread kernel, populations unstaged.
+    const bool staged = bb_b + pop_b + 8 * 64 * sizeof(LineRec) <= 65536;
     dim3 grid((nc + 63) / 64, (ng + VJP_GROUPS - 1) / VJP_GROUPS);
-    if (mixed && staged)
-        hipLaunchKernelGGL((k_vjp_opacity<true, true>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
-    else if (mixed)
-        hipLaunchKernelGGL((k_vjp_opacity<true, false>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
-    else if (staged)
-        hipLaunchKernelGGL((k_vjp_opacity<false, true>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
-    else
-        hipLaunchKernelGGL((k_vjp_opacity<false, false>), grid, dim3(256), lds, c->stream, c->d, flags, ffc1, rgg);
+    if (staged) {
+        if (!(flags & SLGPU_NOBB) && c->nion > 0) {
+            if (mixed)
+                hipLaunchKernelGGL((k_vjp_opacity<true, true, 1, 8>), grid, dim3(512), bb_b + pop_b, c->stream,
+                                   c->d, flags, ffc1, rgg);
+            else
+                hipLaunchKernelGGL((k_vjp_opacity<false, true, 1, 8>), grid, dim3(512), bb_b + pop_b, c->stream,
+                                   c->d, flags, ffc1, rgg);
+        }
+        if (!(flags & SLGPU_NOFF) && c->d.nel > 0) {
+            if (mixed)
+                hipLaunchKernelGGL((k_vjp_opacity<true, false, 2, 4>), grid, dim3(256), ff_b, c->stream,
+                                   c->d, flags, ffc1, rgg);
+            else
+                hipLaunchKernelGGL((k_vjp_opacity<false, false, 2, 4>), grid, dim3(256), ff_b, c->stream,
+                                   c->d, flags, ffc1, rgg);
+        }
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_vjp_opacity<true, false, 3, 4>), grid, dim3(256), acc_b, c->stream, c->d, flags, ffc1, rgg);
+    } else {
+        hipLaunchKernelGGL((k_vjp_opacity<false, false, 3, 4>), grid, dim3(256), acc_b, c->stream, c->d, flags, ffc1, rgg);
+    }
     if (!(flags & SLGPU_NOBF) && c->nion > 0)
         hipLaunchKernelGGL(k_vjp_bf,
                            dim3((nc + 63) / 64, (c->nion + BF_IONS - 1) / BF_IONS, (ng + BF_GROUPS - 1) / BF_GROUPS),
