@@ -179,9 +179,9 @@ def precision_errors(m, f):
 def opacity_grad(args):
     """Differentiation form at the workload's size: forward = device EOS (massfr2natomfr +
     LTE Saha solve) + opacity build + means, backward = opacity VJP + EOS VJP, giving
-    dL/d(T, rho, X) for dL/d(cap, Planck mean, Rosseland mean).  Timed with the library's
-    HIP-event timers: whole phases (including the host transfers of the inputs and of the
-    nc x ng output gradient) and the kernels alone."""
+    dL/d(T, rho, X) for dL/d(cap, Planck mean, Rosseland mean), the output gradients resident
+    on the device.  Timed with the library's HIP-event timers: whole phases (including the
+    host transfers of the per-cell inputs and input gradients) and the kernels alone."""
     from superlite_amd import slgpu, synth
     w, st, wl, lines = synth.build(args.workload)
     tables = synth.load_tables()
@@ -189,15 +189,19 @@ def opacity_grad(args):
     ctx = slgpu.setup_workload(w, st, wl, lines, tables, npartmax=1024, seed=2024, precision=prec)
     ctx.set_eos_data(synth.load_eos_data(args.workload))
     rng = np.random.default_rng(7)
-    gcap = rng.standard_normal((w.nx, w.ng))
-    ones = np.ones(w.nx)
+    # the output gradients live on the device, as a torch caller's would: the backward reads
+    # them in place (slgpu_opacity_vjp_dev), so neither phase moves an nc x ng array over PCIe
+    import torch
+    gcap = torch.from_numpy(rng.standard_normal((w.nx, w.ng))).to("cuda:0")
+    ones = torch.ones(w.nx, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
     nelec = st["nelec"].copy()
 
     def step():
         e = ctx.eos(st["temp"], st["mass"], st["massfr"], nelec=nelec)
         ctx.opacity_build(st["temp"], e["natom"], st["vol"], st["mass"], e["nelec"], e["natomfr"],
                           e["oc"], e["ginv"], want=())
-        g = ctx.opacity_vjp(gcap, ones, ones)
+        g = ctx.opacity_vjp_dev(gcap.data_ptr(), ones.data_ptr(), ones.data_ptr())
         ctx.eos_vjp(g["natom"], g["natomfr"], g["nelec"], g["oc"])
     for _ in range(args.warmup):
         step()

@@ -237,6 +237,18 @@ int slgpu_eos(slgpu_ctx *ctx, const double *temp, const double *mass, const doub
               double acc, double *nelec, double *out_natom, double *out_natomfr,
               double *out_oc, double *out_ginv, int *out_niter);
 
+/* Device-pointer forms of the differentiation form, for callers whose arrays live on the
+ * context's GPU (e.g. torch tensors on that device), so the nc x ng arrays cross no PCIe:
+ * slgpu_opacity_outputs_dev copies the last build's cap64 (ng*nc), capgrey and capross (nc)
+ * into device arrays (any may be NULL); slgpu_opacity_vjp_dev is slgpu_opacity_vjp with the
+ * three input gradients as device arrays (outputs on the host as there).  The library works
+ * on its own stream: the caller finishes writing the input gradients first, and each call
+ * returns with its copies complete.  No counterpart in the reference (like the backward). */
+int slgpu_opacity_outputs_dev(slgpu_ctx *ctx, double *d_cap64, double *d_capgrey, double *d_capross);
+int slgpu_opacity_vjp_dev(slgpu_ctx *ctx, const double *d_grad_cap64, const double *d_grad_capgrey,
+                          const double *d_grad_capross, double *out_grad_temp, double *out_grad_natom,
+                          double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc);
+
 /* eos_update.f:47-58 with in_nlte (nlte_nelem = 1): nlte_ndens(i,1:2,1) of the last slgpu_eos,
  * the H I and H II number densities n_ii natom natomfr(1)/vol from its last Saha evaluation.
  * out_ndens (2*nc, H I then H II per cell; 0 in massless cells) is the ndens argument of

@@ -989,9 +989,12 @@ int slgpu_get_particles(slgpu_ctx *c, int64_t n, uint8_t *stat, double *x, doubl
     return 0;
 }
 
-int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad_capgrey,
-                      const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
-                      double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc)
+namespace {
+// the VJP with its three input gradients on the host (copied in) or, with dev, already on the
+// context's device (read in place)
+int opacity_vjp_impl(slgpu_ctx *c, bool dev, const double *grad_cap64, const double *grad_capgrey,
+                     const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
+                     double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc)
 {
     if (!c) return fail(SLGPU_EINVAL, "null context");
     if (!c->have_opacity) return fail(SLGPU_EORDER, "slgpu_opacity_build must run first");
@@ -1003,18 +1006,21 @@ int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad
     const size_t cg = (size_t)nc * ng;
     double *dgc = nullptr, *dgp = nullptr, *dgr = nullptr, *out = nullptr;
     int rc = 0;
-    if (grad_cap64) rc |= dalloc(c, &dgc, cg);
-    if (grad_capgrey) rc |= dalloc(c, &dgp, nc);
-    if (grad_capross) rc |= dalloc(c, &dgr, nc);
+    if (!dev) {
+        if (grad_cap64) rc |= dalloc(c, &dgc, cg);
+        if (grad_capgrey) rc |= dalloc(c, &dgp, nc);
+        if (grad_capross) rc |= dalloc(c, &dgr, nc);
+    }
     const size_t nout = (size_t)nc * (3 + 30 + std::max(nion, 1));
     rc |= dalloc(c, &out, nout);
     if (rc) return rc;
     if (dgc) HIPCHK(hipMemcpyAsync(dgc, grad_cap64, cg * 8, hipMemcpyHostToDevice, c->stream));
     if (dgp) HIPCHK(hipMemcpyAsync(dgp, grad_capgrey, nc * 8, hipMemcpyHostToDevice, c->stream));
     if (dgr) HIPCHK(hipMemcpyAsync(dgr, grad_capross, nc * 8, hipMemcpyHostToDevice, c->stream));
+    const double *igc = dev ? grad_cap64 : dgc, *igp = dev ? grad_capgrey : dgp, *igr = dev ? grad_capross : dgr;
     double *gT = out, *gn = out + nc, *ge = out + 2 * nc, *gf = out + 3 * nc, *go = out + 33 * (size_t)nc;
     hipEventRecord(c->ev[20], c->stream);
-    launch_opacity_vjp(c, c->opac_flags, dgc, dgp, dgr, gT, gn, ge, gf, go, ff_prefactor(),
+    launch_opacity_vjp(c, c->opac_flags, igc, igp, igr, gT, gn, ge, gf, go, ff_prefactor(),
                        rgg_scratch(c));
     hipEventRecord(c->ev[21], c->stream);
     HIPCHK(hipGetLastError());
@@ -1031,6 +1037,35 @@ int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad
     }
     dfree(c, dgc); dfree(c, dgp); dfree(c, dgr); dfree(c, out);
     return rc;
+}
+}  // namespace
+
+int slgpu_opacity_vjp(slgpu_ctx *c, const double *grad_cap64, const double *grad_capgrey,
+                      const double *grad_capross, double *out_grad_temp, double *out_grad_natom,
+                      double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc)
+{
+    return opacity_vjp_impl(c, false, grad_cap64, grad_capgrey, grad_capross, out_grad_temp,
+                            out_grad_natom, out_grad_nelec, out_grad_natomfr, out_grad_oc);
+}
+
+int slgpu_opacity_vjp_dev(slgpu_ctx *c, const double *d_grad_cap64, const double *d_grad_capgrey,
+                          const double *d_grad_capross, double *out_grad_temp, double *out_grad_natom,
+                          double *out_grad_nelec, double *out_grad_natomfr, double *out_grad_oc)
+{
+    return opacity_vjp_impl(c, true, d_grad_cap64, d_grad_capgrey, d_grad_capross, out_grad_temp,
+                            out_grad_natom, out_grad_nelec, out_grad_natomfr, out_grad_oc);
+}
+
+int slgpu_opacity_outputs_dev(slgpu_ctx *c, double *d_cap64, double *d_capgrey, double *d_capross)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (!c->have_opacity) return fail(SLGPU_EORDER, "an opacity build must run first");
+    const size_t nc = c->nx, cg = nc * c->ng;
+    if (d_cap64) HIPCHK(hipMemcpyAsync(d_cap64, c->d.cap64, cg * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (d_capgrey) HIPCHK(hipMemcpyAsync(d_capgrey, c->d.capgrey, nc * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (d_capross) HIPCHK(hipMemcpyAsync(d_capross, c->d.capross, nc * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int slgpu_sample_groups(slgpu_ctx *c, int table, int direct, int64_t n, const double *r,

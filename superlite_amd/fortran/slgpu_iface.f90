@@ -192,6 +192,24 @@ module slgpu_iface
       type(c_ptr), value :: grad_temp, grad_natom, grad_nelec, grad_natomfr, grad_oc
     end function
 
+    ! the same with the three input gradients as device arrays on the context's GPU
+    integer(c_int) function slgpu_opacity_vjp_dev(ctx, d_grad_cap64, d_grad_capgrey, &
+         d_grad_capross, grad_temp, grad_natom, grad_nelec, grad_natomfr, grad_oc) &
+         bind(C, name="slgpu_opacity_vjp_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: d_grad_cap64, d_grad_capgrey, d_grad_capross
+      type(c_ptr), value :: grad_temp, grad_natom, grad_nelec, grad_natomfr, grad_oc
+    end function
+
+    ! device copies of the last build's cap64, capgrey, capross
+    integer(c_int) function slgpu_opacity_outputs_dev(ctx, d_cap64, d_capgrey, d_capross) &
+         bind(C, name="slgpu_opacity_outputs_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: d_cap64, d_capgrey, d_capross
+    end function
+
     integer(c_int) function slgpu_sample_groups(ctx, table, direct, n, r, cell, group) &
          bind(C, name="slgpu_sample_groups")
       import :: c_int, c_ptr, c_int64_t, c_double

@@ -31,7 +31,7 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
            "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp",
            "slgpu_set_nlte_data", "slgpu_opacity_build_nlte", "slgpu_nlte_rad_est",
-           "slgpu_eos_nlte_ndens")
+           "slgpu_eos_nlte_ndens", "slgpu_opacity_outputs_dev", "slgpu_opacity_vjp_dev")
 
 
 class SlgpuError(RuntimeError):
@@ -55,12 +55,24 @@ class Gas(ctypes.Structure):
 _lib = None
 
 
+def _one_hip_runtime():
+    """torch (ROCm) ships its own libamdhip64 under another file name; loaded after the
+    system one that libslgpu.so names, it would be a second HIP runtime in the process and
+    would find no GPU.  Importing torch first lets libslgpu.so bind to torch's runtime (same
+    soname), so device tensors and the library share one runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libslgpu.so (raises if it is missing: there is no fallback path)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is not built; run python superlite_amd/build.py")
+        _one_hip_runtime()
         L = ctypes.CDLL(LIB)
         L.slgpu_last_error_message.restype = ctypes.c_char_p
         L.slgpu_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(_vp)]
@@ -77,6 +89,8 @@ def lib():
         L.slgpu_opacity_build_nlte.argtypes = [_vp, ctypes.POINTER(Gas), _vp, _vp, _i] + [_vp] * 9
         L.slgpu_nlte_rad_est.argtypes = [_vp, _vp, _vp]
         L.slgpu_eos_nlte_ndens.argtypes = [_vp, _vp]
+        L.slgpu_opacity_outputs_dev.argtypes = [_vp] * 4
+        L.slgpu_opacity_vjp_dev.argtypes = [_vp] * 9
         L.slgpu_ddmc_prep.argtypes = [_vp, _vp, _d, _vp, _vp, _vp, _vp]
         L.slgpu_sources.argtypes = [_vp, _i64, _u32, _vp, _vp]
         L.slgpu_transport.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _vp]
@@ -291,6 +305,25 @@ class Context:
         out["natomfr"] = out["natomfr"].T.copy()
         out["oc"] = out["oc"][:, : self.nion]
         return out
+
+    def opacity_vjp_dev(self, p_grad_cap64=0, p_grad_capgrey=0, p_grad_capross=0):
+        """opacity_vjp with the input gradients given as device addresses on the context's GPU
+        (0 = none; e.g. a contiguous float64 torch tensor's data_ptr(), written and
+        synchronised by the caller)"""
+        nc = self.nx
+        out = dict(temp=np.zeros(nc), natom=np.zeros(nc), nelec=np.zeros(nc),
+                   natomfr=np.zeros((nc, 30)), oc=np.zeros((nc, max(self.nion, 1))))
+        self._check(self.L.slgpu_opacity_vjp_dev(
+            self.h, _vp(p_grad_cap64 or None), _vp(p_grad_capgrey or None), _vp(p_grad_capross or None),
+            *[_p(out[k]) for k in ("temp", "natom", "nelec", "natomfr", "oc")]))
+        out["natomfr"] = out["natomfr"].T.copy()
+        out["oc"] = out["oc"][:, : self.nion]
+        return out
+
+    def opacity_outputs_dev(self, p_cap64=0, p_capgrey=0, p_capross=0):
+        """copy the last build's cap64 (nc, ng), capgrey and capross (nc) to device addresses"""
+        self._check(self.L.slgpu_opacity_outputs_dev(self.h, _vp(p_cap64 or None), _vp(p_capgrey or None),
+                                                     _vp(p_capross or None)))
 
     def set_eos_data(self, ed):
         """the host's LTE EOS data: ni (30), e/q/g1 (30, 31), amass (30)"""

@@ -154,6 +154,45 @@ def test_gas_opacity_gradient_c1(S, tables):
     ctx.close()
 
 
+def test_autograd_device_tensors_c1(S, tables):
+    """the autograd.Functions on device tensors (outputs filled on the device, output
+    gradients read in place) give the host-tensor path's values and gradients"""
+    say("case: autograd with device tensors, c1")
+    import torch
+    from superlite_amd import autograd as sag
+    ctx, w, st, wl, lines = _ctx(S, "c1", tables)
+    rng = np.random.default_rng(43)
+    Wc = rng.standard_normal((w.nx, w.ng))
+    consts = dict(vol=st["vol"], nelec0=st["nelec"], acc=1e-14)
+    rho = st["mass"] / st["vol"]
+    res = {}
+    for where in ("cpu", "cuda:0"):
+        ts = [torch.tensor(a, requires_grad=True, device=where) for a in (st["temp"], rho, st["massfr"])]
+        cap, grey, ross = sag.gas_opacity(ctx, consts, *ts)
+        assert cap.device == ts[0].device and cap.dtype == torch.float64
+        ((cap * torch.from_numpy(Wc).to(where)).sum() + grey.sum() - ross.sum()).backward()
+        res[where] = ([x.detach().cpu().numpy() for x in (cap, grey, ross)],
+                      [t.grad.detach().cpu().numpy() for t in ts])
+        assert all(t.grad.device == ts[0].device for t in ts)
+    for a, b in zip(res["cpu"][0], res["cuda:0"][0]):
+        assert np.array_equal(a, b)
+    for a, b, k in zip(res["cpu"][1], res["cuda:0"][1], ("temp", "rho", "massfr")):
+        np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-300 + 1e-12 * np.abs(a).max(), err_msg=k)
+    # the build-level Function as well
+    e = ctx.eos(st["temp"], st["mass"], st["massfr"], nelec=st["nelec"], acc=1e-14)
+    oc_consts = dict(vol=st["vol"], mass=st["mass"], ginv=e["ginv"])
+    out = {}
+    for where in ("cpu", "cuda:0"):
+        ts = [torch.tensor(a, requires_grad=True, device=where)
+              for a in (st["temp"], e["natom"], e["nelec"], e["natomfr"], e["oc"])]
+        cap, grey, ross = sag.opacity(ctx, oc_consts, *ts)
+        (cap * torch.from_numpy(Wc).to(where)).sum().backward()
+        out[where] = [t.grad.detach().cpu().numpy() for t in ts]
+    for a, b in zip(out["cpu"], out["cuda:0"]):
+        np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-300 + 1e-12 * np.abs(a).max())
+    ctx.close()
+
+
 def test_gas_opacity_gradient_c2_64_probes(S, tables):
     """The differentiation form at the 400-shell size (10^4 groups, 2e6 lines): 64 random
     (shell, parameter) probes over T, rho and the 8 mass fractions, central differences at
