@@ -14,8 +14,8 @@
 //                   walks a copy of the line list ordered by ion within each group, so
 //                   each run of one ion's lines reads its population once and adds to its
 //                   accumulator once
-//   k_vjp_bf        the bf part, a contraction over groups of the cross sections with
-//                   dL/dcap
+//   k_vjp_bf_mfma   the bf part, a contraction over groups of the cross sections with
+//                   dL/dcap, on the FP64 matrix cores
 //   k_vjp_finish    one thread per cell: chain rule through hckt = hc/kT, the ff prefactor
 //                   (natom/V)^2 n_e and sigma_T = c_T n_e natom/V
 //
@@ -133,7 +133,7 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v)
 // line populations of the block's 64 cells (FP32 in the precision form) and each wave's slot
 // of 64 line records.  A population that is not positive contributes nothing (the forward
 // skips it); it is staged as -1 in the precision form, so that a positive one that
-// underflows FP32 (+0) still contributes its derivative, and as 0 in FP64.  The bf part is k_vjp_bf.
+// underflows FP32 (+0) still contributes its derivative, and as 0 in FP64.  The bf part is k_vjp_bf_mfma.
 // P selects the parts: 1 the bb lines, 2 ff, 3 both; NWV waves per block share its 64 cells'
 // accumulators, each wave taking VJP_GROUPS / NWV of the block's groups.
 template <bool M, bool S, int P, int NWV>
@@ -332,35 +332,37 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
     }
 }
 
-// bf part of the backward: d(sum_k bfxs(g,k) oc(k,i))/d(oc(k,i)) summed against G(g,i), a
-// contraction over the groups, goc(k,i) += sum_g bfxs(g,k) G(g,i).  One lane per cell,
-// BF_IONS ions and BF_GROUPS groups per block: the lane keeps its ions' partial sums in
-// registers and reads its cell's G once per group (the group's cross sections are
-// wave-uniform loads); the partial sums of the group chunks meet in FP64 atomics.
-constexpr int BF_IONS = 16, BF_GROUPS = 128;
-__global__ __launch_bounds__(64) void k_vjp_bf(Dev d)
+// bf part of the backward on the matrix cores: d(sum_k bfxs(g,k) oc(k,i))/d(oc(k,i)) summed
+// against G(g,i), goc(k, i) += sum_g bfxs(g, k) G(g, i), is a (nion x ng) x (ng x nc) FP64
+// product.  One wave per 16-ion x 16-cell tile and per slice of
+// the groups (split over waves for parallelism; the partial tiles meet in FP64 atomics), four
+// groups per v_mfma_f64_16x16x4f64: lane l supplies A(ion l%16, group l/16) and
+// B(group l/16, cell l%16), and its accumulator r holds D(ion 4r + l/16, cell l%16) (the
+// instruction's output layout, measured on gfx950).  Out-of-range ions, cells and groups
+// contribute zeros.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int BFM_SLICES = 32;
+__global__ __launch_bounds__(64) void k_vjp_bf_mfma(Dev d)
 {
     const int nc = d.nx, ng = d.ng, nion = d.nion;
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k0 = blockIdx.y * BF_IONS;
-    const int g0 = blockIdx.z * BF_GROUPS, g1 = g0 + BF_GROUPS < ng ? g0 + BF_GROUPS : ng;
-    if (i >= nc) return;
-    double acc[BF_IONS];
-    #pragma unroll
-    for (int q = 0; q < BF_IONS; ++q) acc[q] = 0.0;
-    for (int g = g0; g < g1; ++g) {
-        const double G = d.gcapT[(size_t)g * nc + i];
-        const double *xs = d.bfxs + (size_t)g * nion + k0;
-        #pragma unroll
-        for (int q = 0; q < BF_IONS; ++q)
-            if (k0 + q < nion) {
-                const double x = xs[q];
-                if (x != 0.0) acc[q] += G * x;
-            }
+    const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
+    const int n0 = blockIdx.x * 16, m0 = blockIdx.y * 16;
+    const int per = ((ng + BFM_SLICES - 1) / BFM_SLICES + 3) & ~3;
+    const int g0 = blockIdx.z * per, g1 = g0 + per < ng ? g0 + per : ng;
+    const int ion = m0 + lr, cell = n0 + lr;
+    const bool ion_ok = ion < nion, cell_ok = cell < nc;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int g = g0 + lk; g - lk < g1; g += 4) {
+        const bool g_ok = g < g1;
+        const double a = (ion_ok && g_ok) ? d.bfxs[(size_t)g * nion + ion] : 0.0;
+        const double b = (cell_ok && g_ok) ? d.gcapT[(size_t)g * nc + cell] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     #pragma unroll
-    for (int q = 0; q < BF_IONS; ++q)
-        if (k0 + q < nion && acc[q] != 0.0) atomicAdd(&d.goc[(size_t)(k0 + q) * nc + i], acc[q]);
+    for (int r = 0; r < 4; ++r) {
+        const int k = m0 + 4 * r + lk;
+        if (k < nion && cell_ok && acc[r] != 0.0) atomicAdd(&d.goc[(size_t)k * nc + cell], acc[r]);
+    }
 }
 
 // chain rule to the inputs; outputs in the forward's input layouts
@@ -425,8 +427,7 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
         hipLaunchKernelGGL((k_vjp_opacity<false, false, 3, 4>), grid, dim3(256), acc_b, c->stream, c->d, flags, ffc1, rgg);
     }
     if (!(flags & SLGPU_NOBF) && c->nion > 0)
-        hipLaunchKernelGGL(k_vjp_bf,
-                           dim3((nc + 63) / 64, (c->nion + BF_IONS - 1) / BF_IONS, (ng + BF_GROUPS - 1) / BF_GROUPS),
+        hipLaunchKernelGGL(k_vjp_bf_mfma, dim3((nc + 15) / 16, (c->nion + 15) / 16, BFM_SLICES),
                            dim3(64), 0, c->stream, c->d);
     hipLaunchKernelGGL(k_vjp_finish, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d,
                        flags, gT, gnat, gne, gfr, goc);

@@ -363,6 +363,38 @@ def test_opacity_vjp_central_differences(S, tables):
     ctx.close()
 
 
+@pytest.mark.parametrize("part,flags", [("bf", 1 | 4 | 8), ("bb", 2 | 4 | 8)])
+def test_opacity_vjp_populations_linear(S, tables, part, flags):
+    """With only bf (or only bb) on, the group opacity is linear in the ion populations, so
+    dL/doc(i, k) = sum_g G(i, g) cap_k(i, g) where cap_k is the build with oc = 1 for ion k
+    and 0 for the others: the backward's per-ion gradient against that sum over all ions and
+    cells of c1 (FP64, relative to the largest gradient), and in the precision form.  The bb
+    sum skips lines of ions whose population is not positive (physical_opacity.f:146), so
+    there (hydrogen at c1) the opacity is flat in oc and the gradient is 0."""
+    say(f"case: population gradient of the {part} part against its linear reference, c1")
+    rng = np.random.default_rng(13)
+    for prec, tol in ((S.SLGPU_FP64, 1e-12), (S.SLGPU_MIXED, 1e-5)):
+        ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1024, precision=prec)
+        G = rng.standard_normal((w.nx, w.ng))
+        args = [st["temp"], st["natom"], st["vol"], st["mass"], st["nelec"], st["natomfr"], st["oc"], st["ginv"]]
+        ctx.opacity_build(*args, flags=flags, want=("cap64",))
+        g = ctx.opacity_vjp(G, None, None)["oc"]
+        ref = np.zeros_like(st["oc"])
+        for k in range(st["oc"].shape[1]):
+            oc1 = np.zeros_like(st["oc"])
+            oc1[:, k] = 1.0
+            a = list(args)
+            a[6] = oc1
+            ref[:, k] = (G * ctx.opacity_build(*a, flags=flags, want=("cap64",))["cap64"]).sum(axis=1)
+        ctx.close()
+        live = st["oc"] > 0 if part == "bb" else np.ones(st["oc"].shape, bool)
+        err = np.abs(g - ref)[live].max() / np.abs(ref[live]).max()
+        say(f"{part} precision {prec}: max |goc - ref| / max|ref| = {err:.2e} "
+            f"over {live.sum()} of {live.size} (cell, ion)")
+        assert err < tol
+        assert np.all(g[~live] == 0.0)
+
+
 def test_opacity_vjp_precision_form(S, tables):
     """The precision form's backward (FP32 Kahan line terms) against the FP64 backward on
     the same inputs and output gradients: every input gradient within 1e-5 of the largest
