@@ -321,14 +321,14 @@ def main():
                                    "but not built (run with --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.calibration_copy_mib > 0:
+        calibration_copy(args.calibration_copy_mib)
     if args.mode == "opacity-grad":
         opacity_grad(args)
         return
     if args.mode == "nlte":
         nlte_iteration(args)
         return
-    if args.calibration_copy_mib > 0:
-        calibration_copy(args.calibration_copy_mib)
     from superlite_amd import slgpu, synth
     w, st, wl, lines = synth.build(args.workload)
     tables = synth.load_tables()

@@ -40,8 +40,9 @@ enum {
 };
 
 /* Arithmetic of a context.  SLGPU_FP64: the reference's FP64 (gas_cap is FP32 as in
- * physical_opacity.f:267).  SLGPU_MIXED, the precision form: FP32 Kahan-compensated
- * bound-bound line sums in the opacity build and its backward; FP32 sampled path lengths,
+ * physical_opacity.f:267).  SLGPU_MIXED, the precision form: FP32 bound-bound line
+ * terms, summed in short FP32 partials folded into FP64, in the opacity build and its
+ * backward; FP32 sampled path lengths,
  * event times, adiabatic factors and r^3-uniform positions in the transport.  The
  * shell-crossing quadratic, DDMC leakage thresholds, positions, energies and every tally
  * stay FP64. */

@@ -250,16 +250,9 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// exp(-x) for x >= 0 in FP32, for the precision form's line terms: 2^(-x log2 e) with the
-// rounding error of the product -x*log2(e) (its fma residual plus the tail of log2 e)
-// applied to first order.  About 1.5 ulp, against ~1 ulp for expf at half the instructions.
-__device__ __forceinline__ float exp_neg32(float x)
-{
-    const float y = -x * 1.44269502f;
-    const float e = fmaf(-x, 1.44269502f, -y) + -x * 1.925963e-8f;
-    const float p = __builtin_amdgcn_exp2f(y);
-    return fmaf(p, e * 0.693147181f, p);
-}
+// log2(e): the precision form evaluates exp(-chilw*hckt) as 2^(-chilw * hckt*log2(e)) with
+// hckt*log2(e) rounded once to FP32 per cell (v_exp_f32 computes 2^x)
+constexpr double LOG2E = 1.4426950408889634;
 
 // Block-wide deterministic sum (fixed tree) for 256-thread blocks.
 __device__ __forceinline__ double block_sum256(double v, double *sh)

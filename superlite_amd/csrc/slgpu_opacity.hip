@@ -15,6 +15,14 @@
 
 namespace slgpu {
 
+// One precision-form line term for this lane's cell: gxs * population * exp(-chilw*hckt),
+// the exponential as 2^(-chilw * hl) with hl = hckt*log2(e). oc_lane points at the lane's
+// cell in the block's LDS populations (ion-major, 64 cells per ion).
+__device__ __forceinline__ float line_term32(const LineRec &r, const float *oc_lane, float hl)
+{
+    return r.gxs * oc_lane[r.ion * 64] * __builtin_amdgcn_exp2f(-(r.chilw * hl));
+}
+
 #define PH1(p, s, z, e) ((double)ph1[(p - 1) + 6 * ((s - 1) + 7 * ((z - 1) + 30 * (e - 1)))])
 #define PH2(p, z, e) ((double)ph2[(p - 1) + 7 * ((z - 1) + 30 * (e - 1))])
 // bf_ph1(6,..)**2, bf_ph2(5,..)**2 and bf_ph2(7,..)**2 are single-precision expressions in
@@ -170,44 +178,46 @@ __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
         const int l0 = d.gstart[g], l1 = d.gstart[g + 1];
         const double *ocg = d.ocg + ic;
         if constexpr (M) {
-            // precision form: FP32 terms, Kahan-compensated FP32 sum, widened once
-            const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
-            float sum = 0.0f, comp = 0.0f;
+            // Precision form: FP32 terms gxs * oc * 2^(-chilw * hl), hl = hckt * log2(e) rounded
+            // once from FP64 (one multiply and one v_exp_f32 per line). The terms are all
+            // positive; each chunk of 64 lines is summed in four FP32 partials of at most 16
+            // terms, which are then added in FP64. phi and expfac multiply the FP64 sum once.
+            const float hl32 = (float)(hckt * LOG2E);
+            double sum64 = 0.0;
             if constexpr (S) {
                 LineRec *ln = s_lines[threadIdx.x >> 6];
+                const float *oc_lane = s_oc + lane;
                 for (int base = l0; base < l1; base += 64) {
                     const int n = l1 - base < 64 ? l1 - base : 64;
                     wave_lds_sync();
-                    if (lane < n) {
-                        LineRec r = d.lines[base + lane];
-                        r.gxs = phi32 * r.gxs;                 // the term's first product
-                        ln[lane] = r;
-                    }
+                    if (lane < n) ln[lane] = d.lines[base + lane];
                     wave_lds_sync();
-                    #pragma unroll 4
-                    for (int j = 0; j < n; ++j) {
-                        const LineRec r = ln[j];
-                        const float of = s_oc[r.ion * 64 + lane];
-                        const float t = r.gxs * of * exp_neg32(r.chilw * hk32) * ef32;
-                        const float y = t - comp;
-                        const float u = sum + y;
-                        comp = (u - sum) - y;
-                        sum = u;
+                    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+                    int j = 0;
+                    for (; j + 4 <= n; j += 4) {
+                        s0 += line_term32(ln[j], oc_lane, hl32);
+                        s1 += line_term32(ln[j + 1], oc_lane, hl32);
+                        s2 += line_term32(ln[j + 2], oc_lane, hl32);
+                        s3 += line_term32(ln[j + 3], oc_lane, hl32);
                     }
+                    for (; j < n; ++j) s0 += line_term32(ln[j], oc_lane, hl32);
+                    sum64 += ((double)s0 + (double)s1) + ((double)s2 + (double)s3);
                 }
             } else {
-                for (int l = l0; l < l1; ++l) {
-                    const LineRec r = d.lines[l];
-                    // (float)o > 0 exactly when o > 0 gives a nonzero term
-                    const float of = (float)ocg[(size_t)r.ion * nc];
-                    const float t = phi32 * r.gxs * of * exp_neg32(r.chilw * hk32) * ef32;
-                    const float y = ((of > 0.0f) ? t : 0.0f) - comp;
-                    const float u = sum + y;
-                    comp = (u - sum) - y;
-                    sum = u;
+                for (int base = l0; base < l1; base += 64) {
+                    const int lend = l1 - base < 64 ? l1 : base + 64;
+                    float s0 = 0.0f;
+                    for (int l = base; l < lend; ++l) {
+                        const LineRec r = d.lines[l];
+                        // (float)o > 0 exactly when o > 0 gives a nonzero term
+                        const float of = (float)ocg[(size_t)r.ion * nc];
+                        const float t = r.gxs * of * __builtin_amdgcn_exp2f(-(r.chilw * hl32));
+                        s0 += (of > 0.0f) ? t : 0.0f;
+                    }
+                    sum64 += (double)s0;
                 }
             }
-            acc += (double)sum;
+            acc += sum64 * phi * expfac;
         } else {
             if constexpr (S) {
                 LineRec *ln = s_lines[threadIdx.x >> 6];

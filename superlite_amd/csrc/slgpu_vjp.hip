@@ -21,7 +21,7 @@
 //
 // The means are taken through the FP32-rounded cap as the forward evaluates them, with the
 // rounding treated as the identity.  In the precision form (SLGPU_MIXED) the recomputed
-// bb line terms are FP32 with Kahan-compensated sums, like that form's forward; bf, ff,
+// bb line terms are FP32 in short partial sums folded into FP64, like that form's forward; bf, ff,
 // the means and every accumulator stay FP64.  Elements absent from every cell are skipped
 // by the forward's ff sum, so their mass-fraction gradient is returned as 0.
 #include <type_traits>
@@ -116,17 +116,8 @@ __global__ __launch_bounds__(256) void k_vjp_means(Dev d, const double *__restri
     }
 }
 
-// Kahan-compensated FP32 accumulation (precision form of the line sums)
-__device__ __forceinline__ void kahan_add(float &sum, float &comp, float v)
-{
-    const float y = v - comp;
-    const float t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
-}
-
 // dynamic LDS: s_oc[nion][64] then s_fr[nel][64].  M: the precision form, whose bb line
-// terms are recomputed in FP32 with Kahan-compensated sums as in its forward.
+// terms are recomputed in FP32 as in its forward, in partial sums folded into FP64.
 // bb and ff of the backward.  LDS: per-ion sums of G*base over the block's lines
 // (s_bb; multiplied by ginv at the flush, since d(oc*ginv*base)/d(oc) = ginv*base and ginv
 // is constant over lines and groups), per-element ff sums (s_fr), and with S the staged
@@ -181,61 +172,81 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
             const double *ocg = d.ocg + ic;
             double s0 = 0.0, s1 = 0.0;
             if constexpr (M) {
-                const float phi32 = (float)phi, ef32 = (float)expfac, hk32 = (float)hckt;
-                float a0 = 0.0f, c0 = 0.0f, a1 = 0.0f, c1 = 0.0f;
+                // FP32 line terms as in the forward's precision form: base = gxs * 2^(-chilw*hl),
+                // hl = hckt*log2(e) rounded once from FP64. phi * expfac multiplies the FP64
+                // sums. The terms are positive, and FP32 partial sums of at most 32 lines are
+                // folded into FP64.
+                const float hl32 = (float)(hckt * LOG2E);
+                const double pe = phi * expfac;
+                double t0 = 0.0, t1 = 0.0;
                 if constexpr (S) {
                     // the group's lines by ion: one population load and one accumulator add
-                    // per run of an ion's lines (the run test is wave-uniform)
+                    // per run of an ion's lines (the run test is wave-uniform); a population
+                    // that is not positive (stored as -1) contributes no term and no gradient
                     LineRec *ln = s_lines[wave];
                     int cur = -1;
-                    float ofc = -1.0f;
-                    double sb = 0.0;
+                    bool pop_ok = false;
+                    float ofc = 0.0f, sb = 0.0f;
+                    double sbd = 0.0;
                     for (int base0 = l0; base0 < l1; base0 += 64) {
                         const int n = l1 - base0 < 64 ? l1 - base0 : 64;
                         wave_lds_sync();
-                        if (lane < n) {
-                            LineRec r = d.lines_ion[base0 + lane];
-                            r.gxs = phi32 * r.gxs;             // base's first product
-                            ln[lane] = r;
-                        }
+                        if (lane < n) ln[lane] = d.lines_ion[base0 + lane];
                         wave_lds_sync();
                         // the next record is read while this one is evaluated
                         LineRec rn = ln[0];
-                        for (int j = 0; j < n; ++j) {
-                            const LineRec r = rn;
-                            rn = ln[j + 1 < n ? j + 1 : j];
-                            const int ion = __builtin_amdgcn_readfirstlane(r.ion);
-                            if (ion != cur) {
-                                if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
-                                cur = ion;
-                                sb = 0.0;
-                                ofc = s_pop[ion * 64 + lane];
-                            }
-                            if (ofc >= 0.0f) {
-                                const float base = r.gxs * exp_neg32(r.chilw * hk32) * ef32;
+                        for (int j0 = 0; j0 < n; j0 += 32) {
+                            const int j1 = n - j0 < 32 ? n : j0 + 32;
+                            float a0 = 0.0f, a1 = 0.0f;
+                            for (int j = j0; j < j1; ++j) {
+                                const LineRec r = rn;
+                                rn = ln[j + 1 < n ? j + 1 : j];
+                                const int ion = __builtin_amdgcn_readfirstlane(r.ion);
+                                if (ion != cur) {
+                                    const double sbt = sbd + (double)sb;
+                                    if (pop_ok && sbt != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * pe * sbt);
+                                    cur = ion;
+                                    sb = 0.0f;
+                                    sbd = 0.0;
+                                    const float o = s_pop[ion * 64 + lane];
+                                    pop_ok = o >= 0.0f;
+                                    ofc = pop_ok ? o : 0.0f;
+                                }
+                                const float base = r.gxs * __builtin_amdgcn_exp2f(-(r.chilw * hl32));
                                 const float term = base * ofc;
-                                kahan_add(a0, c0, term);
-                                kahan_add(a1, c1, term * r.chilw);
-                                sb += (double)base;
+                                a0 += term;
+                                a1 = fmaf(term, r.chilw, a1);
+                                sb += base;
                             }
+                            t0 += (double)a0;
+                            t1 += (double)a1;
+                            sbd += (double)sb;
+                            sb = 0.0f;
                         }
                     }
-                    if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
+                    const double sbt = sbd + (double)sb;
+                    if (pop_ok && sbt != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * pe * sbt);
                 } else {
-                    for (int l = l0; l < l1; ++l) {
-                        const LineRec r = d.lines[l];
-                        const double o = ocg[(size_t)r.ion * nc];
-                        if (o > 0.0) {
-                            const float base = phi32 * r.gxs * exp_neg32(r.chilw * hk32) * ef32;
-                            const float term = base * (float)o;
-                            kahan_add(a0, c0, term);
-                            kahan_add(a1, c1, term * r.chilw);
-                            atomicAdd(&s_bb[r.ion * 64 + lane], G * (double)base);
+                    for (int l0b = l0; l0b < l1; l0b += 32) {
+                        const int l1b = l1 - l0b < 32 ? l1 : l0b + 32;
+                        float a0 = 0.0f, a1 = 0.0f;
+                        for (int l = l0b; l < l1b; ++l) {
+                            const LineRec r = d.lines[l];
+                            const double o = ocg[(size_t)r.ion * nc];
+                            if (o > 0.0) {
+                                const float base = r.gxs * __builtin_amdgcn_exp2f(-(r.chilw * hl32));
+                                const float term = base * (float)o;
+                                a0 += term;
+                                a1 = fmaf(term, r.chilw, a1);
+                                atomicAdd(&s_bb[r.ion * 64 + lane], G * pe * (double)base);
+                            }
                         }
+                        t0 += (double)a0;
+                        t1 += (double)a1;
                     }
                 }
-                s0 = (double)a0;
-                s1 = (double)a1;
+                s0 = pe * t0;
+                s1 = pe * t1;
             } else {
                 if constexpr (S) {
                     LineRec *ln = s_lines[wave];

@@ -67,6 +67,33 @@ def coarse(a, nb):
 
 
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("workload", ["c2", "c5"])
+def test_precision_form_opacity_full_size(S, tables, workload):
+    """The precision form's group opacity and means against the FP64 build on the full c2
+    and c5 inputs (c5 holds the dense CSM shell and the widest temperature range), every
+    (cell, group) within 1e-5. The bb line terms are FP32, with the exponential as
+    2^(-chilw * hckt log2 e) and each 64-line chunk summed in four FP32 partials."""
+    say(f"case: precision-form opacity against FP64, {workload} full size")
+    w, st, wl, lines = synth.build(workload)
+    out = {}
+    for prec in (S.SLGPU_FP64, S.SLGPU_MIXED):
+        ctx = S.setup_workload(w, st, wl, lines, tables, precision=prec, seed=3, npartmax=1024)
+        out[prec] = gpu_opacity(ctx, st)
+        ctx.close()
+    o0, o1 = out[S.SLGPU_FP64], out[S.SLGPU_MIXED]
+    assert np.array_equal(o0["cap64"] > 0, o1["cap64"] > 0)
+    m = o0["cap64"] > 0
+    rel = np.abs(o1["cap64"][m] - o0["cap64"][m]) / o0["cap64"][m]
+    rel32 = np.abs(o1["cap"][m].astype(float) - o0["cap"][m]) / o0["cap"][m]
+    say(f"{workload}: cap64 max rel {rel.max():.3e} (median {np.median(rel):.2e}), "
+        f"gas_cap max rel {rel32.max():.3e}")
+    assert rel.max() < 1e-5 and rel32.max() < 1e-5
+    for k in ("capgrey", "capross"):
+        e = np.max(np.abs(o1[k] - o0[k]) / o0[k])
+        say(f"{workload}: {k} max rel {e:.3e}")
+        assert e < 1e-5, k
+
+
 def test_precision_form_c2(S, tables):
     """c3: the precision form on the c2 inputs against FP64 with the same seeds."""
     say("case: precision form c2 (2^16 particles x 8 iterations per precision)")
@@ -193,7 +220,7 @@ def test_opacity_vjp_c2_full_size(S, tables):
     """c4: backward of the full-size opacity build against central differences at
     h = 1e-4 relative on 64 random (shell, parameter) probes: temperature and populations on
     full builds, and the free-free inputs (natom, n_e, mass fractions) on ff-only builds,
-    where their term is resolvable in FP64; then the precision form's backward (FP32 Kahan
+    where their term is resolvable in FP64; then the precision form's backward (FP32
     line terms) against the FP64 backward on the same output gradients."""
     say("case: opacity backward c2 full size")
     w, st, wl, lines = synth.build("c2")

@@ -396,7 +396,7 @@ def test_opacity_vjp_populations_linear(S, tables, part, flags):
 
 
 def test_opacity_vjp_precision_form(S, tables):
-    """The precision form's backward (FP32 Kahan line terms) against the FP64 backward on
+    """The precision form's backward (FP32 line terms) against the FP64 backward on
     the same inputs and output gradients: every input gradient within 1e-5 of the largest
     FP64 gradient of its kind."""
     say("case: opacity backward, precision form c1")
