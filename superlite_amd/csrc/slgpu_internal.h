@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/slgpu.h"
+#include "slgpu_device.h"
 
 namespace slgpu {
 
@@ -56,6 +57,11 @@ struct Dev {
     const double *gff;          // 81*41 Fortran order
     // per-iteration gas state (cell-fastest, [k*nc + i])
     double *hckt, *hlparr, *ocg, *ocT, *sig, *tempinv, *temp, *mass, *natomfr;
+    double *ffB;                          // nx*FF_NU: per cell, the sum over the unclamped elements
+                                          // of z^2 natomfr times g_ff interpolated in gamma^2, at
+                                          // each u node (physical_opacity.f:228-243)
+    uint32_t *ffclamp;                    // nx: bit k set when element k's gamma^2 coordinate lies
+                                          // past the table, where g_ff is clamped to >= 1 per group
     int *el_z;                  // nel present elements (ascending iz)
     // opacity products
     double *capT;               // ng*nc cell-fastest scratch
@@ -202,4 +208,34 @@ void launch_eos_vjp(slgpu_ctx *c, const double *temp, const double *mass, const 
                     const double *oc, const double *g_natom, const double *g_fr,
                     const double *g_ne, const double *g_oc, double *out_T, double *out_mass,
                     double *out_mf);
+
+// physical_opacity.f:228-243: sum over elements of g_ff(u, gamma^2) z^2 natomfr for cell ic at
+// the u coordinate (iu, du). The bilinear g_ff is linear in the table rows of u, so the
+// elements whose gamma^2 coordinate lies inside the table are folded into the cell's row
+// ffB (built each opacity build by k_ff_table), leaving two loads per (cell, group). Elements
+// past the table (bit set in ffclamp) keep the per-group evaluation with g_ff >= 1.
+__device__ inline double ff_gaunt_sum(const Dev &d, int ic, int iu, double du,
+                                      const double *__restrict__ rgg_tab)
+{
+    const int nc = d.nx;
+    const double *B = d.ffB + (size_t)ic * FF_NU;
+    double s = (1.0 - du) * B[iu - 1] + du * B[iu];
+    for (uint32_t m = d.ffclamp[ic]; m != 0u; m &= m - 1u) {
+        const int k = __builtin_ctz(m);
+        const int z = d.el_z[k];
+        const double rgg = rgg_tab[(size_t)k * nc + ic];
+        int igg = (int)floor(rgg);
+        igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
+        const double dgg = rgg - igg;
+        const double *T = d.gff;
+        double gff = (1.0 - du) * (1.0 - dgg) * T[(igg - 1) * FF_NU + (iu - 1)] +
+                     du * (1.0 - dgg) * T[(igg - 1) * FF_NU + iu] +
+                     (1.0 - du) * dgg * T[igg * FF_NU + (iu - 1)] +
+                     du * dgg * T[igg * FF_NU + iu];
+        gff = gff > 1.0 ? gff : 1.0;
+        s += gff * (double)(z * z) * d.natomfr[(size_t)ic * 30 + (z - 1)];
+    }
+    return s;
+}
+
 }  // namespace slgpu

@@ -263,23 +263,10 @@ __global__ __launch_bounds__(256) void k_opacity_nlte(Dev d, NlteDev q, int flag
         iu = iu < 1 ? 1 : (iu > FF_NU - 1 ? FF_NU - 1 : iu);
         const double du = ru - iu;
         const double help = ffc1 * sqrt(hckt) * (1.0 - exp(-u)) * (wl * wl * wl) * d.hlparr[ic];
-        for (int k = 0; k < d.nel; ++k) {
-            const int z = d.el_z[k];
-            const double rgg = rgg_tab[(size_t)k * nc + ic];
-            int igg = (int)floor(rgg);
-            igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
-            const double dgg = rgg - igg;
-            const double *Tg = d.gff;
-            double gff = (1.0 - du) * (1.0 - dgg) * Tg[(igg - 1) * FF_NU + (iu - 1)] +
-                         du * (1.0 - dgg) * Tg[(igg - 1) * FF_NU + iu] +
-                         (1.0 - du) * dgg * Tg[igg * FF_NU + (iu - 1)] +
-                         du * dgg * Tg[igg * FF_NU + iu];
-            if (rgg > FF_NGG) gff = gff > 1.0 ? gff : 1.0;
-            const double v = help * gff * (double)(z * z) * d.natomfr[(size_t)ic * 30 + (z - 1)];
-            cap += v;
-            cape += v;
-            em += v * bnul;
-        }
+        const double v = help * ff_gaunt_sum(d, ic, iu, du, rgg_tab);
+        cap += v;
+        cape += v;
+        em += v * bnul;
     }
     if (act) {
         const size_t o = (size_t)(g - 1) * nc + i;

@@ -126,6 +126,30 @@ __global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const doubl
     }
 }
 
+// The cell's free-free row (physical_opacity.f:228-243): for each u node j,
+// ffB[ic][j] = sum over the elements with gamma^2 coordinate rgg <= FF_NGG of
+// z^2 natomfr * ((1 - dgg) g_ff[igg-1][j] + dgg g_ff[igg][j]); elements past the table go to
+// the cell's ffclamp mask. One block of 128 threads per cell, thread j < FF_NU.
+__global__ __launch_bounds__(128) void k_ff_table(Dev d, const double *__restrict__ rgg_tab)
+{
+    const int nc = d.nx, ic = blockIdx.x, j = threadIdx.x;
+    if (j >= FF_NU) return;
+    double b = 0.0;
+    uint32_t m = 0u;
+    for (int k = 0; k < d.nel; ++k) {
+        const double rgg = rgg_tab[(size_t)k * nc + ic];
+        if (rgg > FF_NGG) { m |= 1u << k; continue; }
+        const int z = d.el_z[k];
+        int igg = (int)floor(rgg);
+        igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
+        const double dgg = rgg - igg;
+        const double w = (double)(z * z) * d.natomfr[(size_t)ic * 30 + (z - 1)];
+        b += w * ((1.0 - dgg) * d.gff[(igg - 1) * FF_NU + j] + dgg * d.gff[igg * FF_NU + j]);
+    }
+    d.ffB[(size_t)ic * FF_NU + j] = b;
+    if (j == 0) d.ffclamp[ic] = m;
+}
+
 // Fused bb + bf + ff for one (cell, group); result into capT[g*nc + i] (cell-fastest).
 // blockDim = 256 = 64 cells x 4 groups; grid = (ceil(nc/64), ceil(ng/4)).
 // S: the block's 64 cells' line populations oc*ginv of every ion are staged in LDS once
@@ -262,20 +286,7 @@ __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
         iu = iu < 1 ? 1 : (iu > FF_NU - 1 ? FF_NU - 1 : iu);
         const double du = ru - iu;
         const double help = ffc1 * sqrt(hckt) * (1.0 - exp(-u)) * (wl * wl * wl) * d.hlparr[ic];
-        for (int k = 0; k < d.nel; ++k) {
-            const int z = d.el_z[k];
-            const double rgg = rgg_tab[(size_t)k * nc + ic];
-            int igg = (int)floor(rgg);
-            igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
-            const double dgg = rgg - igg;
-            const double *T = d.gff;
-            double gff = (1.0 - du) * (1.0 - dgg) * T[(igg - 1) * FF_NU + (iu - 1)] +
-                         du * (1.0 - dgg) * T[(igg - 1) * FF_NU + iu] +
-                         (1.0 - du) * dgg * T[igg * FF_NU + (iu - 1)] +
-                         du * dgg * T[igg * FF_NU + iu];
-            if (rgg > FF_NGG) gff = gff > 1.0 ? gff : 1.0;
-            acc += help * gff * (double)(z * z) * d.natomfr[(size_t)ic * 30 + (z - 1)];
-        }
+        acc += help * ff_gaunt_sum(d, ic, iu, du, rgg_tab);
     }
     if (act) d.capT[(size_t)(g - 1) * nc + i] = massive ? acc : 0.0;
 }
@@ -359,6 +370,9 @@ void launch_cell_prep(slgpu_ctx *c, const double *d_natom, const double *d_vol,
     int n = c->nx * nk;
     hipLaunchKernelGGL(k_cell_prep, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d,
                        d_natom, d_vol, d_nelec, d_oc, d_ginv, flags, g_rgg_scratch(c));
+    if (!(flags & SLGPU_NOFF))
+        hipLaunchKernelGGL(k_ff_table, dim3(c->nx), dim3(128), 0, c->stream, c->d,
+                           (const double *)g_rgg_scratch(c));
 }
 
 // free-free prefactor of physical_opacity.f:51-52 (cgs)
