@@ -212,6 +212,8 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.hlparr, nc);
     rc |= dalloc(c, &d.ffB, (size_t)nc * FF_NU);
     rc |= dalloc(c, &d.ffclamp, nc);
+    rc |= dalloc(c, &d.ffC, (size_t)nc * FF_NU);
+    rc |= dalloc(c, &d.ffW, (size_t)nc * FF_NU);
     rc |= dalloc(c, &d.sig, nc);
     rc |= dalloc(c, &d.tempinv, nc);
     rc |= dalloc(c, &d.temp, nc);

@@ -57,11 +57,15 @@ struct Dev {
     const double *gff;          // 81*41 Fortran order
     // per-iteration gas state (cell-fastest, [k*nc + i])
     double *hckt, *hlparr, *ocg, *ocT, *sig, *tempinv, *temp, *mass, *natomfr;
-    double *ffB;                          // nx*FF_NU: per cell, the sum over the unclamped elements
+    double *ffB;                          // FF_NU*nx, cell fastest: per cell, the sum over the unclamped elements
                                           // of z^2 natomfr times g_ff interpolated in gamma^2, at
                                           // each u node (physical_opacity.f:228-243)
     uint32_t *ffclamp;                    // nx: bit k set when element k's gamma^2 coordinate lies
                                           // past the table, where g_ff is clamped to >= 1 per group
+    double *ffC;                          // FF_NU*nx: as ffB, with the g_ff rows' difference in
+                                          // gamma^2 (the backward's slope of g_ff in rgg)
+    double *ffW;                          // FF_NU*nx: backward, sum over groups of G*help spread
+                                          // onto the two u nodes of each group
     int *el_z;                  // nel present elements (ascending iz)
     // opacity products
     double *capT;               // ng*nc cell-fastest scratch
@@ -218,8 +222,8 @@ __device__ inline double ff_gaunt_sum(const Dev &d, int ic, int iu, double du,
                                       const double *__restrict__ rgg_tab)
 {
     const int nc = d.nx;
-    const double *B = d.ffB + (size_t)ic * FF_NU;
-    double s = (1.0 - du) * B[iu - 1] + du * B[iu];
+    const double *B = d.ffB + ic;                 // node-major: B[j * nc]
+    double s = (1.0 - du) * B[(size_t)(iu - 1) * nc] + du * B[(size_t)iu * nc];
     for (uint32_t m = d.ffclamp[ic]; m != 0u; m &= m - 1u) {
         const int k = __builtin_ctz(m);
         const int z = d.el_z[k];

@@ -128,13 +128,14 @@ __global__ void k_cell_prep(Dev d, const double *__restrict__ natom, const doubl
 
 // The cell's free-free row (physical_opacity.f:228-243): for each u node j,
 // ffB[ic][j] = sum over the elements with gamma^2 coordinate rgg <= FF_NGG of
-// z^2 natomfr * ((1 - dgg) g_ff[igg-1][j] + dgg g_ff[igg][j]); elements past the table go to
-// the cell's ffclamp mask. One block of 128 threads per cell, thread j < FF_NU.
+// z^2 natomfr * ((1 - dgg) g_ff[igg-1][j] + dgg g_ff[igg][j]), and ffC[ic][j] the same sum
+// of z^2 natomfr * (g_ff[igg][j] - g_ff[igg-1][j]) for the backward; elements past the table
+// go to the cell's ffclamp mask. One block of 128 threads per cell, thread j < FF_NU.
 __global__ __launch_bounds__(128) void k_ff_table(Dev d, const double *__restrict__ rgg_tab)
 {
     const int nc = d.nx, ic = blockIdx.x, j = threadIdx.x;
     if (j >= FF_NU) return;
-    double b = 0.0;
+    double b = 0.0, cs = 0.0;
     uint32_t m = 0u;
     for (int k = 0; k < d.nel; ++k) {
         const double rgg = rgg_tab[(size_t)k * nc + ic];
@@ -144,9 +145,12 @@ __global__ __launch_bounds__(128) void k_ff_table(Dev d, const double *__restric
         igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
         const double dgg = rgg - igg;
         const double w = (double)(z * z) * d.natomfr[(size_t)ic * 30 + (z - 1)];
-        b += w * ((1.0 - dgg) * d.gff[(igg - 1) * FF_NU + j] + dgg * d.gff[igg * FF_NU + j]);
+        const double lo = d.gff[(igg - 1) * FF_NU + j], hi = d.gff[igg * FF_NU + j];
+        b += w * ((1.0 - dgg) * lo + dgg * hi);
+        cs += w * (hi - lo);
     }
-    d.ffB[(size_t)ic * FF_NU + j] = b;
+    d.ffB[(size_t)j * nc + ic] = b;
+    d.ffC[(size_t)j * nc + ic] = cs;
     if (j == 0) d.ffclamp[ic] = m;
 }
 

@@ -156,6 +156,8 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
     __syncthreads();
     const double hckt = d.hckt[ic];
     double dh_acc = 0.0, dhl_acc = 0.0;
+    int w_iu = 0;                                  // u node pair of the W run (0: none)
+    double w_lo = 0.0, w_hi = 0.0;
     for (int q = 0; q < VJP_GROUPS / NWV; ++q) {
         const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * VJP_GROUPS + wave * (VJP_GROUPS / NWV) + q) + 1;
         if (g > ng) break;
@@ -306,11 +308,35 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
             const double help = hnh * d.hlparr[ic];
             const double dhelp = help * (0.5 / hckt + eu * (1.0 / wl) / (1.0 - eu));
             const double dru = 10.0 / (hckt * LN10), drgg = 5.0 / (hckt * LN10);
-            for (int k = 0; k < nel; ++k) {
+            // Elements inside the g_ff table, through the cell's rows (k_ff_table): the sum
+            // fs = sum_k g_ff zz nf and its slope fsd in hckt (through ru and rgg).
+            const double *B = d.ffB + ic, *C = d.ffC + ic;    // node-major, cell fastest
+            const double b0 = B[(size_t)(iu - 1) * nc], b1 = B[(size_t)iu * nc];
+            double fs = (1.0 - du) * b0 + du * b1;
+            double fsd = (b1 - b0) * dru + ((1.0 - du) * C[(size_t)(iu - 1) * nc] + du * C[(size_t)iu * nc]) * drgg;
+            // Their mass-fraction gradient, sum over groups of G help g_ff zz, is
+            // zz * sum_j A_k(j) W(j), with A_k element k's g_ff row interpolated in rgg and W
+            // the groups' G*help spread over their two u nodes. W accumulates in registers
+            // while iu stays, and goes to ffW when it changes (k_vjp_ff_fr finishes the sum).
+            if (act) {
+                if (iu != w_iu) {
+                    if (w_iu > 0) {
+                        atomicAdd(&d.ffW[(size_t)(w_iu - 1) * nc + ic], w_lo);
+                        atomicAdd(&d.ffW[(size_t)w_iu * nc + ic], w_hi);
+                    }
+                    w_iu = iu;
+                    w_lo = 0.0;
+                    w_hi = 0.0;
+                }
+                w_lo += G * help * (1.0 - du);
+                w_hi += G * help * du;
+            }
+            // Elements past the table, per group, with the reference's g_ff >= 1.
+            for (uint32_t m = d.ffclamp[ic]; m != 0u; m &= m - 1u) {
+                const int k = __builtin_ctz(m);
                 const int z = d.el_z[k];
                 const double rgg = rgg_tab[(size_t)k * nc + ic];
-                int igg = (int)floor(rgg);
-                igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
+                const int igg = FF_NGG - 1;                    // rgg > FF_NGG
                 const double dgg = rgg - igg;
                 const double *T = d.gff;
                 const double t00 = T[(igg - 1) * FF_NU + (iu - 1)], t10 = T[(igg - 1) * FF_NU + iu];
@@ -319,14 +345,20 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
                              (1.0 - du) * dgg * t01 + du * dgg * t11;
                 double dg = ((1.0 - dgg) * (t10 - t00) + dgg * (t11 - t01)) * dru +
                             ((1.0 - du) * (t01 - t00) + du * (t11 - t10)) * drgg;
-                if (rgg > FF_NGG && !(gff > 1.0)) { gff = 1.0; dg = 0.0; }
+                if (!(gff > 1.0)) { gff = 1.0; dg = 0.0; }
                 const double zz = (double)(z * z);
                 const double nf = d.natomfr[(size_t)ic * 30 + (z - 1)];
-                dh_acc += G * (dhelp * gff + help * dg) * zz * nf;
-                dhl_acc += G * hnh * gff * zz * nf;
+                fs += gff * zz * nf;
+                fsd += dg * zz * nf;
                 atomicAdd(&s_fr[k * 64 + lane], G * help * gff * zz);
             }
+            dh_acc += G * (dhelp * fs + help * fsd);
+            dhl_acc += G * hnh * fs;
         }
+    }
+    if (FF && act && w_iu > 0) {
+        atomicAdd(&d.ffW[(size_t)(w_iu - 1) * nc + ic], w_lo);
+        atomicAdd(&d.ffW[(size_t)w_iu * nc + ic], w_hi);
     }
     if (act) {
         if (dh_acc != 0.0) atomicAdd(&d.gcell[2 * nc + i], dh_acc);
@@ -398,6 +430,29 @@ __global__ __launch_bounds__(256) void k_vjp_finish(Dev d, int flags, double *gT
     for (int k = 0; k < d.nion; ++k) goc_out[(size_t)i * d.nion + k] = d.goc[(size_t)k * nc + i];
 }
 
+// The mass-fraction gradient of the elements inside the g_ff table (physical_opacity.f:
+// 228-243): gfr[k][i] += z^2 * sum_j A_k(j) W(i, j), with A_k(j) = (1 - dgg) g_ff[igg-1][j] +
+// dgg g_ff[igg][j] element k's row at the cell's rgg and W the backward's G*help node sums.
+// One thread per (element, cell); elements past the table were summed per group.
+__global__ __launch_bounds__(256) void k_vjp_ff_fr(Dev d, const double *__restrict__ rgg_tab)
+{
+    const int nc = d.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nel * nc) return;
+    const int k = t / nc, i = t - k * nc;
+    const double rgg = rgg_tab[(size_t)k * nc + i];
+    if (rgg > FF_NGG) return;
+    int igg = (int)floor(rgg);
+    igg = igg < 1 ? 1 : (igg > FF_NGG - 1 ? FF_NGG - 1 : igg);
+    const double dgg = rgg - igg;
+    const double *lo = d.gff + (size_t)(igg - 1) * FF_NU, *hi = d.gff + (size_t)igg * FF_NU;
+    const double *W = d.ffW + i;                       // node-major, cell fastest
+    double s = 0.0;
+    for (int j = 0; j < FF_NU; ++j) s += ((1.0 - dgg) * lo[j] + dgg * hi[j]) * W[(size_t)j * nc];
+    const int z = d.el_z[k];
+    d.gfr[(size_t)k * nc + i] += (double)(z * z) * s;
+}
+
 void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double *gp,
                         const double *gr, double *gT, double *gnat, double *gne, double *gfr,
                         double *goc, double ffc1, const double *rgg)
@@ -405,6 +460,7 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     const int nc = c->nx, ng = c->ng;
     (void)hipMemsetAsync(c->d.goc, 0, (size_t)std::max(c->nion, 1) * nc * 8, c->stream);
     (void)hipMemsetAsync(c->d.gfr, 0, (size_t)30 * nc * 8, c->stream);
+    (void)hipMemsetAsync(c->d.ffW, 0, (size_t)FF_NU * nc * 8, c->stream);
     hipLaunchKernelGGL(k_vjp_means, dim3(nc), dim3(256), 0, c->stream, c->d, gc, gp, gr);
     const bool mixed = c->d.precision == SLGPU_MIXED;
     const size_t acc_b = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
@@ -437,6 +493,9 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     } else {
         hipLaunchKernelGGL((k_vjp_opacity<false, false, 3, 4>), grid, dim3(256), acc_b, c->stream, c->d, flags, ffc1, rgg);
     }
+    if (!(flags & SLGPU_NOFF) && c->d.nel > 0)
+        hipLaunchKernelGGL(k_vjp_ff_fr, dim3((c->d.nel * nc + 255) / 256), dim3(256), 0, c->stream,
+                           c->d, rgg);
     if (!(flags & SLGPU_NOBF) && c->nion > 0)
         hipLaunchKernelGGL(k_vjp_bf_mfma, dim3((nc + 15) / 16, (c->nion + 15) / 16, BFM_SLICES),
                            dim3(64), 0, c->stream, c->d);
