@@ -73,6 +73,10 @@ __device__ bool saha_nelec(const Dev &d, const double *fr, double kti, double sa
 }
 
 // flags: bit 0 a cell's solve stopped (nelec <= 0 or invalid Saha normalisation)
+// One 64-lane wave per cell. Lane z-1 evaluates element z's Saha recursion (the 30
+// recursions of one evaluation run in parallel); the wave then adds the 30 terms
+// fr(z)*ne(z) from LDS in the reference's order with the serial loop's expression, so every lane holds the same sum, the
+// secant steps run wave-uniform and the result is the serial one bit for bit.
 __global__ __launch_bounds__(64) void k_eos(Dev d, const double *__restrict__ temp,
                                             const double *__restrict__ mass,
                                             const double *__restrict__ massfr, double acc,
@@ -80,30 +84,32 @@ __global__ __launch_bounds__(64) void k_eos(Dev d, const double *__restrict__ te
                                             double *__restrict__ natomfr, double *__restrict__ oc,
                                             double *__restrict__ ginv, int *__restrict__ niter)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double s_f[30], s_n[30];
+    const int i = blockIdx.x, lane = threadIdx.x;
     if (i >= d.nx) return;
     double *fr = natomfr + (size_t)i * 30;
-    niter[i] = 0;
-    d.eos_sahac2[i] = 0.0;
+    if (lane == 0) {
+        niter[i] = 0;
+        d.eos_sahac2[i] = 0.0;
+    }
     if (mass[i] <= 0.0) {
-        for (int k = 0; k < d.nion; ++k) {
+        for (int k = lane; k < d.nion; k += 64) {
             oc[(size_t)i * d.nion + k] = 0.0;
             ginv[(size_t)i * d.nion + k] = 0.0;
         }
         return;
     }
-    // massfr2natomfr (gas_setup.f:125-157), no container fraction
+    // massfr2natomfr (gas_setup.f:125-157), no container fraction; every lane forms the same
+    // serial sums, and the wave's fractions are kept in registers of every lane (lf)
     const double *mf = massfr + (size_t)i * 30;
     double help = 0.0;
     for (int l = 0; l < 30; ++l) help = help + mf[l];
-    for (int l = 0; l < 30; ++l) {
-        fr[l] = (mf[l] / help) * mass[i];
-        fr[l] = fr[l] / (d.eos_amass[l] * PC_AMU);
-    }
     double na = 0.0;
-    for (int l = 0; l < 30; ++l) na = na + fr[l];
-    natom[i] = na;
-    for (int l = 0; l < 30; ++l) fr[l] = fr[l] / na;
+    for (int l = 0; l < 30; ++l) na = na + (mf[l] / help) * mass[i] / (d.eos_amass[l] * PC_AMU);
+    const int z = lane + 1;                                  // this lane's element (z <= 30)
+    const double frz = z <= 30 ? ((mf[z - 1] / help) * mass[i] / (d.eos_amass[z - 1] * PC_AMU)) / na : 0.0;
+    if (z <= 30) fr[z - 1] = frz;
+    if (lane == 0) natom[i] = na;
     // ions_solve_eos (ionsmod.f:160-309)
     const double T = temp[i];
     const double ndens = na / d.vol[i];
@@ -114,8 +120,26 @@ __global__ __launch_bounds__(64) void k_eos(Dev d, const double *__restrict__ te
     int iprev = 0, iconv;
     for (iconv = 1; iconv <= 40; ++iconv) {
         sahac2 = 2.0 / (ne * ndens) * sahac;
-        double ne_new;
-        if (!saha_nelec(d, fr, kti, sahac2, ne_new)) { atomicOr(d.flags, 1); return; }
+        // saha_nelec: element z's term fr(z) * sum_ii (ii-1) n_ii on lane z-1
+        bool ok = true;
+        if (z <= 30) {
+            double n[31];
+            for (int k = 0; k < 31; ++k) n[k] = 0.0;
+            ok = saha_element(d, z, kti, sahac2, n);
+            double nez = 0.0;
+            for (int ii = 2; ii <= d.eos_ni[z - 1]; ++ii) nez = nez + (ii - 1) * n[ii - 1];
+            s_f[z - 1] = frz;
+            s_n[z - 1] = nez;
+        }
+        const bool bad = __any(!ok);
+        __syncthreads();
+        if (bad) {
+            if (lane == 0) atomicOr(d.flags, 1);
+            return;
+        }
+        double ne_new = 0.0;
+        for (int l = 0; l < 30; ++l) ne_new = ne_new + s_f[l] * s_n[l];
+        __syncthreads();                                     // s_n is rewritten next evaluation
         const double err = ne / ne_new - 1.0;
         if (fabs(err) < acc) {
             if (iconv > 1) ne = ne_new;
@@ -135,24 +159,26 @@ __global__ __launch_bounds__(64) void k_eos(Dev d, const double *__restrict__ te
             iprev = ih;
             const double dxdy = (nel1 - nel0) / (err1 - err0);
             ne = nel0 - dxdy * (err0 - ynew);
-            if (ne <= 0.0) { atomicOr(d.flags, 1); return; }
+            if (ne <= 0.0) {
+                if (lane == 0) atomicOr(d.flags, 1);
+                return;
+            }
         }
     }
-    nelec[i] = ne;
-    niter[i] = iconv;
-    d.eos_sahac2[i] = sahac2;
-    // eos_update.f:36-45: populations of the last Saha evaluation, for the listed ions
-    double n[31];
-    int zlast = 0;
-    for (int k = 0; k < d.nion; ++k) {
+    if (lane == 0) {
+        nelec[i] = ne;
+        niter[i] = iconv;
+        d.eos_sahac2[i] = sahac2;
+    }
+    // eos_update.f:36-45: populations of the last Saha evaluation, one listed ion per lane
+    for (int k = lane; k < d.nion; k += 64) {
         const int iz = d.ion_z[k], ii = d.ion_i[k];
-        if (iz != zlast) {
-            for (int m = 0; m < 31; ++m) n[m] = 0.0;
-            saha_element(d, iz, kti, sahac2, n);
-            zlast = iz;
-        }
+        double n[31];
+        for (int m = 0; m < 31; ++m) n[m] = 0.0;
+        saha_element(d, iz, kti, sahac2, n);
+        const double friz = ((mf[iz - 1] / help) * mass[i] / (d.eos_amass[iz - 1] * PC_AMU)) / na;
         ginv[(size_t)i * d.nion + k] = 1.0 / ED_G(iz, ii);
-        oc[(size_t)i * d.nion + k] = na * fr[iz - 1] * ED_G(iz, ii) * n[ii - 1] / (ED_Q(iz, ii) * d.vol[i]);
+        oc[(size_t)i * d.nion + k] = na * friz * ED_G(iz, ii) * n[ii - 1] / (ED_Q(iz, ii) * d.vol[i]);
     }
 }
 
@@ -174,32 +200,33 @@ __global__ __launch_bounds__(64) void k_eos_vjp(Dev d, const double *__restrict_
                                                 double *__restrict__ out_mass,
                                                 double *__restrict__ out_mf)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // One 64-lane wave per cell: lane z-1 forms element z's stage moments, then lane k the
+    // listed ion k's terms (dynamic LDS, 4 per ion), and the sums run in the serial order.
+    extern __shared__ double s_ion[];
+    __shared__ double s_ma[30], s_mE[30], s_var[30], s_cov[30], s_gfr[30];
+    const int i = blockIdx.x, lane = threadIdx.x;
     if (i >= d.nx) return;
-    out_T[i] = 0.0;
-    out_mass[i] = 0.0;
-    for (int l = 0; l < 30; ++l) out_mf[(size_t)i * 30 + l] = 0.0;
+    if (lane == 0) {
+        out_T[i] = 0.0;
+        out_mass[i] = 0.0;
+    }
+    if (lane < 30) out_mf[(size_t)i * 30 + lane] = 0.0;
     if (mass[i] <= 0.0 || d.eos_sahac2[i] == 0.0) return;
     const double T = temp[i], ne = nelec[i], na = natom[i], V = d.vol[i];
     const double ndens = na / V;
     const double kti = 1.0 / (PC_KB * T);
     const double sahac2 = d.eos_sahac2[i];
     const double *fr = natomfr + (size_t)i * 30;
-    double gna = g_natom ? g_natom[i] : 0.0;
-    double gfr[30];
-    for (int l = 0; l < 30; ++l) gfr[l] = g_fr ? g_fr[(size_t)i * 30 + l] : 0.0;
-    double gne = g_ne ? g_ne[i] : 0.0;
-    double gT = 0.0, glu = 0.0, gnd = 0.0;
-    double sfv = 0.0, sfdT = 0.0;
-    double nez[30];
-    double n[31];
-    int k = 0;
-    for (int iz = 1; iz <= 30; ++iz) {
+    const int nion = d.nion;
+    // element z = lane + 1: the moments of its stage distribution
+    if (lane < 30) {
+        const int iz = lane + 1;
+        double n[31];
         for (int m = 0; m < 31; ++m) n[m] = 0.0;
         saha_element(d, iz, kti, sahac2, n);
-        const int nion = d.eos_ni[iz - 1];
+        const int ni = d.eos_ni[iz - 1];
         double ma = 0.0, ma2 = 0.0, mE = 0.0, maE = 0.0, Ei = 0.0;
-        for (int ii = 1; ii <= nion; ++ii) {
+        for (int ii = 1; ii <= ni; ++ii) {
             if (ii > 1) Ei += ED_E(iz, ii - 1);
             const double a = ii - 1;
             ma += a * n[ii - 1];
@@ -207,52 +234,85 @@ __global__ __launch_bounds__(64) void k_eos_vjp(Dev d, const double *__restrict_
             mE += Ei * n[ii - 1];
             maE += a * Ei * n[ii - 1];
         }
-        const double var = ma2 - ma * ma, cov = maE - ma * mE;
-        nez[iz - 1] = ma;
-        sfv += fr[iz - 1] * var;
-        sfdT += fr[iz - 1] * ((1.5 / T) * var + (kti / T) * cov);
-        // populations of this element's listed ions (the ion list is ascending in iz)
-        for (; k < d.nion && d.ion_z[k] == iz; ++k) {
-            const double go = g_oc ? g_oc[(size_t)i * d.nion + k] : 0.0;
-            if (go == 0.0) continue;
-            const int ii = d.ion_i[k];
+        s_ma[lane] = ma;
+        s_mE[lane] = mE;
+        s_var[lane] = ma2 - ma * ma;
+        s_cov[lane] = maE - ma * mE;
+    }
+    __syncthreads();
+    // listed ion k: its terms of dL/dnatom, dL/dfr(iz), the ln(u) and T adjoints
+    for (int k = lane; k < nion; k += 64) {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        const double go = g_oc ? g_oc[(size_t)i * nion + k] : 0.0;
+        if (go != 0.0) {
+            const int iz = d.ion_z[k], ii = d.ion_i[k];
+            double n[31];
+            for (int m = 0; m < 31; ++m) n[m] = 0.0;
+            saha_element(d, iz, kti, sahac2, n);
             double Ek = 0.0;
             for (int j = 1; j < ii; ++j) Ek += ED_E(iz, j);
-            const double a = ii - 1;
+            const double a = ii - 1, ma = s_ma[iz - 1], mE = s_mE[iz - 1];
             const double base = ED_G(iz, ii) * n[ii - 1] / (ED_Q(iz, ii) * V);   // oc / (natom fr)
-            const double w = go * oc[(size_t)i * d.nion + k];
-            gna += go * fr[iz - 1] * base;
-            gfr[iz - 1] += go * na * base;
-            glu += -w * (a - ma);
-            gT += w * ((1.5 / T) * (a - ma) + (kti / T) * (Ek - mE));
+            const double w = go * oc[(size_t)i * nion + k];
+            t0 = go * fr[iz - 1] * base;
+            t1 = go * na * base;
+            t2 = -w * (a - ma);
+            t3 = w * ((1.5 / T) * (a - ma) + (kti / T) * (Ek - mE));
         }
+        s_ion[4 * k] = t0;
+        s_ion[4 * k + 1] = t1;
+        s_ion[4 * k + 2] = t2;
+        s_ion[4 * k + 3] = t3;
+    }
+    __syncthreads();
+    // the serial sums (the ion list ascends in iz), on every lane
+    double gna = g_natom ? g_natom[i] : 0.0;
+    double gT = 0.0, glu = 0.0, gnd = 0.0, sfv = 0.0, sfdT = 0.0;
+    for (int l = 0; l < 30; ++l) {
+        sfv += fr[l] * s_var[l];
+        sfdT += fr[l] * ((1.5 / T) * s_var[l] + (kti / T) * s_cov[l]);
+    }
+    for (int k = 0; k < nion; ++k) {
+        gna += s_ion[4 * k];
+        glu += s_ion[4 * k + 2];
+        gT += s_ion[4 * k + 3];
     }
     // u = nelec * ndens, then the implicit solution nelec(ndens, T, fr)
+    double gne = g_ne ? g_ne[i] : 0.0;
     gne += glu / ne;
     gnd += glu / ndens;
     const double lam = gne / (1.0 + sfv / ne);
     gnd += -lam * sfv / ndens;
     gT += lam * sfdT;
-    for (int l = 0; l < 30; ++l) gfr[l] += lam * nez[l];
+    if (lane < 30) {
+        double g = g_fr ? g_fr[(size_t)i * 30 + lane] : 0.0;
+        for (int k = 0; k < nion; ++k)
+            if (d.ion_z[k] == lane + 1) g += s_ion[4 * k + 1];
+        s_gfr[lane] = g + lam * s_ma[lane];
+    }
+    __syncthreads();
     gna += gnd / V;
     // massfr2natomfr: fr = y/natom, natom = sum y, y_l = x_l mass/(A_l amu), x = mf/sum(mf)
     const double *mf = massfr + (size_t)i * 30;
     double H = 0.0;
     for (int l = 0; l < 30; ++l) H += mf[l];
     double sfg = 0.0;
-    for (int l = 0; l < 30; ++l) sfg += gfr[l] * fr[l];
+    for (int l = 0; l < 30; ++l) sfg += s_gfr[l] * fr[l];
     const double gnat = gna - sfg / na;            // total adjoint of natom
-    double gx[30], sxg = 0.0, gm = 0.0;
+    double sxg = 0.0, gm = 0.0, gxl = 0.0;
     for (int l = 0; l < 30; ++l) {
-        const double gy = gfr[l] / na + gnat;
+        const double gy = s_gfr[l] / na + gnat;
         const double c = mass[i] / (d.eos_amass[l] * PC_AMU);
-        gx[l] = gy * c;
+        const double gx = gy * c;
+        if (l == lane) gxl = gx;
         gm += gy * (mf[l] / H) / (d.eos_amass[l] * PC_AMU);
-        sxg += gx[l] * (mf[l] / H);
+        sxg += gx * (mf[l] / H);
     }
-    for (int l = 0; l < 30; ++l) out_mf[(size_t)i * 30 + l] = (gx[l] - sxg) / H;
-    out_mass[i] = gm;
-    out_T[i] = gT;
+    if (lane < 30) out_mf[(size_t)i * 30 + lane] = (gxl - sxg) / H;
+    if (lane == 0) {
+        out_mass[i] = gm;
+        out_T[i] = gT;
+    }
 }
 
 // eos_update.f:47-58 with in_nlte: nlte_ndens(i,1:2,1), the H I and H II number densities
@@ -287,7 +347,7 @@ void launch_eos(slgpu_ctx *c, const double *temp, const double *mass, const doub
                 double acc, double *nelec, double *natom, double *natomfr, double *oc,
                 double *ginv, int *niter)
 {
-    hipLaunchKernelGGL(k_eos, dim3((c->nx + 63) / 64), dim3(64), 0, c->stream, c->d, temp, mass,
+    hipLaunchKernelGGL(k_eos, dim3(c->nx), dim3(64), 0, c->stream, c->d, temp, mass,
                        massfr, acc, nelec, natom, natomfr, oc, ginv, niter);
 }
 
@@ -297,7 +357,8 @@ void launch_eos_vjp(slgpu_ctx *c, const double *temp, const double *mass, const 
                     const double *g_ne, const double *g_oc, double *out_T, double *out_mass,
                     double *out_mf)
 {
-    hipLaunchKernelGGL(k_eos_vjp, dim3((c->nx + 63) / 64), dim3(64), 0, c->stream, c->d, temp,
+    hipLaunchKernelGGL(k_eos_vjp, dim3(c->nx), dim3(64), (size_t)4 * (c->nion > 0 ? c->nion : 1) * sizeof(double),
+                       c->stream, c->d, temp,
                        mass, massfr, nelec, natom, natomfr, oc, g_natom, g_fr, g_ne, g_oc, out_T,
                        out_mass, out_mf);
 }
