@@ -64,6 +64,8 @@ struct Dev {
                                           // past the table, where g_ff is clamped to >= 1 per group
     double *ffC;                          // FF_NU*nx: as ffB, with the g_ff rows' difference in
                                           // gamma^2 (the backward's slope of g_ff in rgg)
+    double *gmpart;                       // 2*((ng+63)/64)*nx: the means backward's per-tile
+                                          // partial sums of dL/dT and dL/dsig
     double *ffW;                          // FF_NU*nx: backward, sum over groups of G*help spread
                                           // onto the two u nodes of each group
     int *el_z;                  // nel present elements (ascending iz)

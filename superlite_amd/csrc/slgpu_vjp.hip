@@ -69,46 +69,75 @@ __device__ __forceinline__ double d_spectintv_dT(double tinv, const double *__re
     return r != r ? 0.0 : r;
 }
 
-__device__ __forceinline__ double block_sum256_v(double v, double *sh)
-{
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    const double r = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return r;
-}
-
+// One tile of 16 cells x 64 groups per block. Each wave takes 4 of the tile's cells in
+// turn, its lanes 64 consecutive groups, so cap32 and gc are read along the group axis; the
+// dL/dcap values go through an LDS tile and are stored along the cell axis of gcapT. The
+// per-cell sums of dL/dT and dL/dsig are wave sums in a fixed order, one partial per tile
+// (gmpart), added in tile order by k_vjp_means_sum.
 __global__ __launch_bounds__(256) void k_vjp_means(Dev d, const double *__restrict__ gc,
                                                    const double *__restrict__ gp,
                                                    const double *__restrict__ gr)
 {
-    __shared__ double sh[4];
-    const int i = blockIdx.x, ng = d.ng, nc = d.nx;
-    const double tinv = d.tempinv[i], sg = d.sig[i], R = d.capross[i];
-    const double wp = gp ? gp[i] : 0.0, wr = gr ? gr[i] : 0.0;
-    const bool massive = d.mass[i] > 0.0;
-    double dT = 0.0, dsig = 0.0;
-    for (int g = threadIdx.x + 1; g <= ng; g += 256) {
-        const double c = (double)d.cap32[(size_t)i * ng + g - 1];
-        const double s = specintv_g(tinv, d.wlinv, g);
-        const double t = spectintv_g(tinv, d.wlinv, g);
-        const double w = sg + c;
-        double G = gc ? gc[(size_t)i * ng + g - 1] : 0.0;
-        G += wp * s;
-        dT += wp * c * d_specintv_dT(tinv, d.wlinv, g);
-        if (w > 0.0 && R != 0.0) {
-            // capross = 1/sum_g t_g/(sig + cap_g)
-            G += wr * R * R * t / (w * w);
-            dsig += wr * R * R * t / (w * w);
-            dT -= wr * R * R * d_spectintv_dT(tinv, d.wlinv, g) / w;
+    __shared__ double s_o[64][17];
+    const int ng = d.ng, nc = d.nx;
+    const int c0 = blockIdx.x * 16, g0 = blockIdx.y * 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = g0 + lane + 1;                       // this lane's group (1-based)
+    const bool gok = g <= ng;
+    const int nty = gridDim.y;
+    for (int cl = wave * 4; cl < wave * 4 + 4; ++cl) {
+        const int i = c0 + cl;
+        if (i >= nc) break;                            // wave-uniform
+        const double tinv = d.tempinv[i], sg = d.sig[i], R = d.capross[i];
+        const double wp = gp ? gp[i] : 0.0, wr = gr ? gr[i] : 0.0;
+        const bool massive = d.mass[i] > 0.0;
+        double dT = 0.0, dsig = 0.0, G = 0.0;
+        if (gok) {
+            const double c = (double)d.cap32[(size_t)i * ng + g - 1];
+            const double s = specintv_g(tinv, d.wlinv, g);
+            const double t = spectintv_g(tinv, d.wlinv, g);
+            const double w = sg + c;
+            G = gc ? gc[(size_t)i * ng + g - 1] : 0.0;
+            G += wp * s;
+            dT += wp * c * d_specintv_dT(tinv, d.wlinv, g);
+            if (w > 0.0 && R != 0.0) {
+                // capross = 1/sum_g t_g/(sig + cap_g)
+                G += wr * R * R * t / (w * w);
+                dsig += wr * R * R * t / (w * w);
+                dT -= wr * R * R * d_spectintv_dT(tinv, d.wlinv, g) / w;
+            }
+            if (!massive) G = 0.0;
         }
-        d.gcapT[(size_t)(g - 1) * nc + i] = massive ? G : 0.0;
+        s_o[lane][cl] = G;
+        dT = wave_sum(dT);
+        dsig = wave_sum(dsig);
+        if (lane == 0) {
+            d.gmpart[((size_t)0 * nty + blockIdx.y) * nc + i] = dT;
+            d.gmpart[((size_t)1 * nty + blockIdx.y) * nc + i] = dsig;
+        }
     }
-    dT = block_sum256_v(dT, sh);
-    dsig = block_sum256_v(dsig, sh);
-    if (threadIdx.x == 0) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < 64 * 16; k += 256) {
+        const int gl = k >> 4, cl = k & 15;
+        if (g0 + gl < ng && c0 + cl < nc) d.gcapT[(size_t)(g0 + gl) * nc + c0 + cl] = s_o[gl][cl];
+    }
+}
+
+// dL/dT and dL/dsig of the means: one wave per cell adds the tiles' partials (lane y takes
+// tiles y, y+64, ...; then a fixed-order wave sum); clears the slots the line and ff kernels
+// add into
+__global__ __launch_bounds__(64) void k_vjp_means_sum(Dev d, int nty)
+{
+    const int i = blockIdx.x, lane = threadIdx.x, nc = d.nx;
+    if (i >= nc) return;
+    double dT = 0.0, dsig = 0.0;
+    for (int y = lane; y < nty; y += 64) {
+        dT += d.gmpart[((size_t)0 * nty + y) * nc + i];
+        dsig += d.gmpart[((size_t)1 * nty + y) * nc + i];
+    }
+    dT = wave_sum(dT);
+    dsig = wave_sum(dsig);
+    if (lane == 0) {
         d.gcell[0 * nc + i] = dT;
         d.gcell[1 * nc + i] = dsig;
         d.gcell[2 * nc + i] = 0.0;
@@ -461,7 +490,11 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     (void)hipMemsetAsync(c->d.goc, 0, (size_t)std::max(c->nion, 1) * nc * 8, c->stream);
     (void)hipMemsetAsync(c->d.gfr, 0, (size_t)30 * nc * 8, c->stream);
     (void)hipMemsetAsync(c->d.ffW, 0, (size_t)FF_NU * nc * 8, c->stream);
-    hipLaunchKernelGGL(k_vjp_means, dim3(nc), dim3(256), 0, c->stream, c->d, gc, gp, gr);
+    {
+        const int nty = (ng + 63) / 64;
+        hipLaunchKernelGGL(k_vjp_means, dim3((nc + 15) / 16, nty), dim3(256), 0, c->stream, c->d, gc, gp, gr);
+        hipLaunchKernelGGL(k_vjp_means_sum, dim3(nc), dim3(64), 0, c->stream, c->d, nty);
+    }
     const bool mixed = c->d.precision == SLGPU_MIXED;
     const size_t acc_b = (size_t)(c->nion + c->d.nel) * 64 * sizeof(double);
     const size_t bb_b = (size_t)c->nion * 64 * sizeof(double), ff_b = (size_t)c->d.nel * 64 * sizeof(double);
