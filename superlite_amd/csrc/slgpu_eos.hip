@@ -7,9 +7,10 @@
 //   k_eos_vjp  the adjoint of k_eos at its converged solution
 //   k_eos_ndens the H I / H II number densities of the last solve (eos_update.f:47-58)
 //
-// The work is tiny (at most 8192 cells, each ~40 x 30 Saha recursions), so one thread runs
-// one cell in the reference's exact order of operations; the partition functions are the
-// host's, frozen at its first EOS call (ionsmod.f:208-215).
+// One 64-lane wave runs one cell: lane z-1 takes element z's Saha recursion (and, for the
+// populations, lane k the listed ion k), and every sum runs in the reference's serial order
+// from LDS, so the results are those of the one-thread-per-cell order of operations; the
+// partition functions are the host's, frozen at its first EOS call (ionsmod.f:208-215).
 //
 // Adjoint.  With u = nelec * ndens (the electron density; sahac2 = 2 sahac(T)/u), stage i of
 // element z has the Saha weight log p_i = (i-1) log sahac2 - E_i/kT + log(q_i/q_1), E_i the

@@ -4,8 +4,11 @@
 // with respect to the build's own inputs: temperature, natom, n_e, the mass fractions and
 // the ion populations oc (ginv, V and the masses are constants of the build).
 //
-//   k_vjp_means     one block per cell: the means' gradients folded into dL/dcap, plus
-//                   their direct T and sigma_Thomson terms
+//   k_vjp_means     16 cells x 64 groups per block: the means' gradients folded into
+//                   dL/dcap, plus their direct T and sigma_Thomson terms as per-tile partials
+//   k_vjp_means_sum one wave per cell: the tiles' partials of those terms
+//   k_vjp_ff_fr     one thread per (element, cell): the ff mass-fraction gradient of the
+//                   elements inside the g_ff table, from the per-cell node sums
 //   k_vjp_opacity   one thread per (cell, group), 64 cells x 16 groups per block: the bb
 //                   line terms recomputed (not stored) and ff, with per-ion and
 //                   per-element accumulators in LDS flushed once per block; bb in
