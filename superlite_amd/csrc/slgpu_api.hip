@@ -470,6 +470,12 @@ static int upload_lines(slgpu_ctx *c)
     for (int q = 1; q <= ng; ++q)
         std::stable_sort(recs_ion.begin() + gstart[q], recs_ion.begin() + gstart[q + 1],
                          [](const LineRec &a, const LineRec &b) { return a.ion < b.ion; });
+    // pad of an ion-ordered record: the lines left in its run of one ion within its group
+    // (itself included), so the backward walks a run as a counted loop
+    for (int q = 1; q <= ng; ++q)
+        for (int l = gstart[q + 1] - 1; l >= gstart[q]; --l)
+            recs_ion[l].pad = (l + 1 < gstart[q + 1] && recs_ion[l + 1].ion == recs_ion[l].ion)
+                                  ? recs_ion[l + 1].pad + 1 : 1;
     dfree(c, (void *)c->d.lines);
     dfree(c, (void *)c->d.lines_ion);
     dfree(c, (void *)c->d.gstart);

@@ -17,7 +17,7 @@ struct LineRec {
     float gxs;      // g*f*(pi e^2/m_e c)
     float chilw;    // lower-level energy [1/cm]
     int ion;        // index into the population list
-    int pad;
+    int pad;        // in the ion-ordered copy: lines left in this record's ion run
 };
 
 // Per-cell DDMC constants (diffusion11.f90:95-152), evaluated once per iteration.

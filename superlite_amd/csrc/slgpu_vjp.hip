@@ -227,25 +227,27 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
                         wave_lds_sync();
                         if (lane < n) ln[lane] = d.lines_ion[base0 + lane];
                         wave_lds_sync();
-                        // the next record is read while this one is evaluated
-                        LineRec rn = ln[0];
-                        for (int j0 = 0; j0 < n; j0 += 32) {
-                            const int j1 = n - j0 < 32 ? n : j0 + 32;
+                        // one run of an ion's lines at a time (pad: lines left in the run),
+                        // at most 32 lines per FP32 partial, as a counted loop
+                        for (int j = 0; j < n;) {
+                            const int ion = __builtin_amdgcn_readfirstlane(ln[j].ion);
+                            int len = __builtin_amdgcn_readfirstlane(ln[j].pad);
+                            len = len < n - j ? len : n - j;
+                            len = len < 32 ? len : 32;
+                            len = len > 1 ? len : 1;                 // always advances
+                            if (ion != cur) {
+                                const double sbt = sbd + (double)sb;
+                                if (pop_ok && sbt != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * pe * sbt);
+                                cur = ion;
+                                sb = 0.0f;
+                                sbd = 0.0;
+                                const float o = s_pop[ion * 64 + lane];
+                                pop_ok = o >= 0.0f;
+                                ofc = pop_ok ? o : 0.0f;
+                            }
                             float a0 = 0.0f, a1 = 0.0f;
-                            for (int j = j0; j < j1; ++j) {
-                                const LineRec r = rn;
-                                rn = ln[j + 1 < n ? j + 1 : j];
-                                const int ion = __builtin_amdgcn_readfirstlane(r.ion);
-                                if (ion != cur) {
-                                    const double sbt = sbd + (double)sb;
-                                    if (pop_ok && sbt != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * pe * sbt);
-                                    cur = ion;
-                                    sb = 0.0f;
-                                    sbd = 0.0;
-                                    const float o = s_pop[ion * 64 + lane];
-                                    pop_ok = o >= 0.0f;
-                                    ofc = pop_ok ? o : 0.0f;
-                                }
+                            for (int t = j; t < j + len; ++t) {
+                                const LineRec r = ln[t];
                                 const float base = r.gxs * __builtin_amdgcn_exp2f(-(r.chilw * hl32));
                                 const float term = base * ofc;
                                 a0 += term;
@@ -256,6 +258,7 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
                             t1 += (double)a1;
                             sbd += (double)sb;
                             sb = 0.0f;
+                            j += len;
                         }
                     }
                     const double sbt = sbd + (double)sb;
@@ -291,11 +294,12 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
                         wave_lds_sync();
                         if (lane < n) ln[lane] = d.lines_ion[base0 + lane];
                         wave_lds_sync();
-                        LineRec rn = ln[0];
-                        for (int j = 0; j < n; ++j) {
-                            const LineRec r = rn;
-                            rn = ln[j + 1 < n ? j + 1 : j];
-                            const int ion = __builtin_amdgcn_readfirstlane(r.ion);
+                        // one run of an ion's lines at a time (pad: lines left in the run)
+                        for (int j = 0; j < n;) {
+                            const int ion = __builtin_amdgcn_readfirstlane(ln[j].ion);
+                            int len = __builtin_amdgcn_readfirstlane(ln[j].pad);
+                            len = len < n - j ? len : n - j;
+                            len = len > 1 ? len : 1;                 // always advances
                             if (ion != cur) {
                                 if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
                                 cur = ion;
@@ -303,12 +307,16 @@ void k_vjp_opacity(Dev d, int flags, double ffc1, const double *__restrict__ rgg
                                 oc_ = s_pop[ion * 64 + lane];
                             }
                             if (oc_ > 0.0) {
-                                const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
-                                const double term = base * oc_;
-                                s0 += term;
-                                s1 += term * (double)r.chilw;
-                                sb += base;
+                                for (int t = j; t < j + len; ++t) {
+                                    const LineRec r = ln[t];
+                                    const double base = phi * (double)r.gxs * exp(-((double)r.chilw * hckt)) * expfac;
+                                    const double term = base * oc_;
+                                    s0 += term;
+                                    s1 += term * (double)r.chilw;
+                                    sb += base;
+                                }
                             }
+                            j += len;
                         }
                     }
                     if (sb != 0.0) atomicAdd(&s_bb[cur * 64 + lane], G * sb);
