@@ -15,14 +15,6 @@
 
 namespace slgpu {
 
-// One precision-form line term for this lane's cell: gxs * population * exp(-chilw*hckt),
-// the exponential as 2^(-chilw * hl) with hl = hckt*log2(e). oc_lane points at the lane's
-// cell in the block's LDS populations (ion-major, 64 cells per ion).
-__device__ __forceinline__ float line_term32(const LineRec &r, const float *oc_lane, float hl)
-{
-    return r.gxs * oc_lane[r.ion * 64] * __builtin_amdgcn_exp2f(-(r.chilw * hl));
-}
-
 #define PH1(p, s, z, e) ((double)ph1[(p - 1) + 6 * ((s - 1) + 7 * ((z - 1) + 30 * (e - 1)))])
 #define PH2(p, z, e) ((double)ph2[(p - 1) + 7 * ((z - 1) + 30 * (e - 1))])
 // bf_ph1(6,..)**2, bf_ph2(5,..)**2 and bf_ph2(7,..)**2 are single-precision expressions in
@@ -208,28 +200,38 @@ __global__ __launch_bounds__(256) void k_opacity(Dev d, int flags, double ffc1,
         if constexpr (M) {
             // Precision form: FP32 terms gxs * oc * 2^(-chilw * hl), hl = hckt * log2(e) rounded
             // once from FP64 (one multiply and one v_exp_f32 per line). The terms are all
-            // positive; each chunk of 64 lines is summed in four FP32 partials of at most 16
-            // terms, which are then added in FP64. phi and expfac multiply the FP64 sum once.
+            // positive; each run of one ion's lines (at most 64, within a chunk) is summed in
+            // two FP32 partials, multiplied by the population and added in FP64. phi and
+            // expfac multiply the FP64 sum once.
             const float hl32 = (float)(hckt * LOG2E);
             double sum64 = 0.0;
             if constexpr (S) {
+                // the group's lines in ion order (Dev::lines_ion, whose pad holds the lines left
+                // in the record's ion run): per run one population read, per line
+                // gxs * 2^(-chilw * hl) into two FP32 run sums of at most 32 terms each
                 LineRec *ln = s_lines[threadIdx.x >> 6];
                 const float *oc_lane = s_oc + lane;
                 for (int base = l0; base < l1; base += 64) {
                     const int n = l1 - base < 64 ? l1 - base : 64;
                     wave_lds_sync();
-                    if (lane < n) ln[lane] = d.lines[base + lane];
+                    if (lane < n) ln[lane] = d.lines_ion[base + lane];
                     wave_lds_sync();
-                    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-                    int j = 0;
-                    for (; j + 4 <= n; j += 4) {
-                        s0 += line_term32(ln[j], oc_lane, hl32);
-                        s1 += line_term32(ln[j + 1], oc_lane, hl32);
-                        s2 += line_term32(ln[j + 2], oc_lane, hl32);
-                        s3 += line_term32(ln[j + 3], oc_lane, hl32);
+                    for (int j = 0; j < n;) {
+                        const int ion = __builtin_amdgcn_readfirstlane(ln[j].ion);
+                        int len = __builtin_amdgcn_readfirstlane(ln[j].pad);
+                        len = len < n - j ? len : n - j;
+                        len = len > 1 ? len : 1;                     // always advances
+                        const float of = oc_lane[ion * 64];
+                        float r0 = 0.0f, r1 = 0.0f;
+                        int t = j;
+                        for (; t + 2 <= j + len; t += 2) {
+                            r0 = fmaf(ln[t].gxs, __builtin_amdgcn_exp2f(-(ln[t].chilw * hl32)), r0);
+                            r1 = fmaf(ln[t + 1].gxs, __builtin_amdgcn_exp2f(-(ln[t + 1].chilw * hl32)), r1);
+                        }
+                        if (t < j + len) r0 = fmaf(ln[t].gxs, __builtin_amdgcn_exp2f(-(ln[t].chilw * hl32)), r0);
+                        sum64 += (double)((r0 + r1) * of);
+                        j += len;
                     }
-                    for (; j < n; ++j) s0 += line_term32(ln[j], oc_lane, hl32);
-                    sum64 += ((double)s0 + (double)s1) + ((double)s2 + (double)s3);
                 }
             } else {
                 for (int base = l0; base < l1; base += 64) {
