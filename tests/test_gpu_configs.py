@@ -72,7 +72,8 @@ def test_precision_form_opacity_full_size(S, tables, workload):
     """The precision form's group opacity and means against the FP64 build on the full c2
     and c5 inputs (c5 holds the dense CSM shell and the widest temperature range), every
     (cell, group) within 1e-5. The bb line terms are FP32, with the exponential as
-    2^(-chilw * hckt log2 e) and each 64-line chunk summed in four FP32 partials."""
+    2^(-chilw * hckt log2 e), summed per run of one ion's lines in FP32 run sums that are
+    multiplied by the population and added in FP64."""
     say(f"case: precision-form opacity against FP64, {workload} full size")
     w, st, wl, lines = synth.build(workload)
     out = {}
