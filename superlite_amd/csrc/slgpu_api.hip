@@ -243,6 +243,7 @@ int slgpu_create(const slgpu_config *cfg, slgpu_ctx **out)
     rc |= dalloc(c, &d.gcapT, cg);
     rc |= dalloc(c, &d.gcell, 4 * nc);
     rc |= dalloc(c, &d.gmpart, (size_t)2 * ((ng + 63) / 64) * nc);
+    rc |= dalloc(c, &d.ctpart, (size_t)2 * ((ng + 1023) / 1024) * nc);
     rc |= dalloc(c, &d.gfr, 30 * nc);
     rc |= dalloc(c, &d.cprob, nc);
     rc |= dalloc(c, &d.gprob, cg);

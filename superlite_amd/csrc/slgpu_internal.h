@@ -64,6 +64,7 @@ struct Dev {
                                           // past the table, where g_ff is clamped to >= 1 per group
     double *ffC;                          // FF_NU*nx: as ffB, with the g_ff rows' difference in
                                           // gamma^2 (the backward's slope of g_ff in rgg)
+    double *ctpart;                       // 2*((ng+1023)/1024)*nx: the means' per-slice partial sums
     double *gmpart;                       // 2*((ng+63)/64)*nx: the means backward's per-tile
                                           // partial sums of dL/dT and dL/dsig
     double *ffW;                          // FF_NU*nx: backward, sum over groups of G*help spread

@@ -330,14 +330,18 @@ __global__ __launch_bounds__(256) void k_opacity_finish(Dev d, int want64)
 
 // Planck group integrals specintv(1/T) for the DDMC/emission tables, and the Planck and
 // Rosseland means of the FP32 group opacity.  One 256-thread block per cell.
+// One block per cell and slice of 1024 groups: specarr, and the slice's partial sums of the
+// Planck and Rosseland means (fixed-order block sums) into ctpart; k_cell_tables_sum adds the
+// slices in a fixed order.
 __global__ __launch_bounds__(256) void k_cell_tables(Dev d)
 {
     __shared__ double sh[4];
-    const int i = blockIdx.x, ng = d.ng;
+    const int i = blockIdx.x, ng = d.ng, nc = d.nx, ns = gridDim.y;
     const double tinv = d.tempinv[i];
     const double sg = d.sig[i];
+    const int gend = (int)blockIdx.y * 1024 + 1024 < ng ? (int)blockIdx.y * 1024 + 1024 : ng;
     double pg = 0.0, pr = 0.0;
-    for (int g = threadIdx.x + 1; g <= ng; g += 256) {
+    for (int g = blockIdx.y * 1024 + threadIdx.x + 1; g <= gend; g += 256) {
         const double c = (double)d.cap32[(size_t)i * ng + g - 1];
         const double s = specintv_g(tinv, d.wlinv, g);
         d.specarr[(size_t)i * ng + g - 1] = s;
@@ -347,6 +351,25 @@ __global__ __launch_bounds__(256) void k_cell_tables(Dev d)
     pg = block_sum256(pg, sh);
     pr = block_sum256(pr, sh);
     if (threadIdx.x == 0) {
+        d.ctpart[((size_t)0 * ns + blockIdx.y) * nc + i] = pg;
+        d.ctpart[((size_t)1 * ns + blockIdx.y) * nc + i] = pr;
+    }
+}
+
+// The means from the slices' partials: one wave per cell, lane y takes slices y, y+64, ...,
+// then a fixed-order wave sum.
+__global__ __launch_bounds__(64) void k_cell_tables_sum(Dev d, int ns)
+{
+    const int i = blockIdx.x, lane = threadIdx.x, nc = d.nx;
+    if (i >= nc) return;
+    double pg = 0.0, pr = 0.0;
+    for (int y = lane; y < ns; y += 64) {
+        pg += d.ctpart[((size_t)0 * ns + y) * nc + i];
+        pr += d.ctpart[((size_t)1 * ns + y) * nc + i];
+    }
+    pg = wave_sum(pg);
+    pr = wave_sum(pr);
+    if (lane == 0) {
         d.capgrey[i] = pg;
         d.capross[i] = 1.0 / pr;
     }
@@ -424,7 +447,9 @@ void launch_opacity_finish(slgpu_ctx *c)
 
 void launch_cell_tables(slgpu_ctx *c)
 {
-    hipLaunchKernelGGL(k_cell_tables, dim3(c->nx), dim3(256), 0, c->stream, c->d);
+    const int ns = (c->ng + 1023) / 1024;
+    hipLaunchKernelGGL(k_cell_tables, dim3(c->nx, ns), dim3(256), 0, c->stream, c->d);
+    hipLaunchKernelGGL(k_cell_tables_sum, dim3(c->nx), dim3(64), 0, c->stream, c->d, ns);
 }
 
 }  // namespace slgpu
