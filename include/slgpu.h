@@ -266,6 +266,13 @@ int slgpu_eos_vjp(slgpu_ctx *ctx, const double *grad_natom, const double *grad_n
                   const double *grad_nelec, const double *grad_oc, double *out_grad_temp,
                   double *out_grad_mass, double *out_grad_massfr);
 
+/* Verification hook, no counterpart in the reference: on = 1 makes the context's opacity
+ * build and its backward take the kernels they use when the ion populations do not fit in
+ * LDS (more than 256 ions for the build in the precision form, 128 in FP64; about 74 and 56
+ * for the backward), so tests can compare them with the staged kernels on small inputs.
+ * on = 0 (the default) restores the choice by size. */
+int slgpu_set_unstaged(slgpu_ctx *ctx, int on);
+
 /* Verification hook for the transport's group samplers; no single reference routine
  * corresponds.  It evaluates, for n (draw r, cell) pairs, the sampler the transport
  * uses: table 0-3 = the lumped loops of diffusion11.f90 (left leak :375-398, right leak

@@ -1080,6 +1080,14 @@ int slgpu_opacity_outputs_dev(slgpu_ctx *c, double *d_cap64, double *d_capgrey, 
     return 0;
 }
 
+int slgpu_set_unstaged(slgpu_ctx *c, int on)
+{
+    if (!c) return fail(SLGPU_EINVAL, "null context");
+    if (on != 0 && on != 1) return fail(SLGPU_EINVAL, "on must be 0 or 1");
+    c->force_unstaged = on;
+    return SLGPU_OK;
+}
+
 int slgpu_sample_groups(slgpu_ctx *c, int table, int direct, int64_t n, const double *r,
                         const int *cell, int *out_group)
 {

@@ -157,6 +157,7 @@ struct slgpu_ctx {
     bool have_grid = false, have_groups = false, have_lines = false, have_bf = false,
          have_ff = false, have_ions = false, have_opacity = false, have_prep = false,
          have_sources = false, have_transport = false;
+    int force_unstaged = 0;                // slgpu_set_unstaged: opacity and backward unstaged
     int64_t nline = 0, nline_used = 0;
     std::vector<double> h_xarr, h_vxarr, h_divv, h_vol, h_rcell, h_wl;
     std::vector<int> h_ion_iz, h_ion_ii;

@@ -423,7 +423,7 @@ void launch_opacity(slgpu_ctx *c, int flags)
     // staged populations: nion x 64 cells, up to 64 KiB of LDS per block beside the 4 KiB
     // of line records
     const size_t lds = (size_t)c->nion * 64 * (mixed ? sizeof(float) : sizeof(double));
-    const bool staged = c->nion > 0 && lds <= 65536;
+    const bool staged = !c->force_unstaged && c->nion > 0 && lds <= 65536;
     if (mixed && staged)
         hipLaunchKernelGGL((k_opacity<true, true>), grid, dim3(256), lds, c->stream, c->d, flags,
                            ffc1, rgg);

@@ -513,7 +513,7 @@ void launch_opacity_vjp(slgpu_ctx *c, int flags, const double *gc, const double 
     // With the populations staged and the line slots of 8 waves within 64 KiB of LDS, the bb
     // lines run in 512-thread blocks (8 waves x 2 groups sharing the cells' accumulators) and
     // ff in a kernel of its own; otherwise both in one 256-thread kernel, populations unstaged.
-    const bool staged = bb_b + pop_b + 8 * 64 * sizeof(LineRec) <= 65536;
+    const bool staged = !c->force_unstaged && bb_b + pop_b + 8 * 64 * sizeof(LineRec) <= 65536;
     dim3 grid((nc + 63) / 64, (ng + VJP_GROUPS - 1) / VJP_GROUPS);
     if (staged) {
         if (!(flags & SLGPU_NOBB) && c->nion > 0) {

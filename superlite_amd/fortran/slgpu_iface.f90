@@ -210,6 +210,12 @@ module slgpu_iface
       type(c_ptr), value :: d_cap64, d_capgrey, d_capross
     end function
 
+    integer(c_int) function slgpu_set_unstaged(ctx, on) bind(C, name="slgpu_set_unstaged")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+
     integer(c_int) function slgpu_sample_groups(ctx, table, direct, n, r, cell, group) &
          bind(C, name="slgpu_sample_groups")
       import :: c_int, c_ptr, c_int64_t, c_double

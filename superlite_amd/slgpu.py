@@ -31,7 +31,8 @@ EXPORTS = ("slgpu_create", "slgpu_destroy", "slgpu_last_error_message", "slgpu_s
            "slgpu_opacity_vjp", "slgpu_timers", "slgpu_stream", "slgpu_peak_rates",
            "slgpu_opacity_build_noalloc", "slgpu_set_eos_data", "slgpu_eos", "slgpu_eos_vjp",
            "slgpu_set_nlte_data", "slgpu_opacity_build_nlte", "slgpu_nlte_rad_est",
-           "slgpu_eos_nlte_ndens", "slgpu_opacity_outputs_dev", "slgpu_opacity_vjp_dev")
+           "slgpu_eos_nlte_ndens", "slgpu_opacity_outputs_dev", "slgpu_opacity_vjp_dev",
+           "slgpu_set_unstaged")
 
 
 class SlgpuError(RuntimeError):
@@ -97,6 +98,7 @@ def lib():
         L.slgpu_fluxtally.argtypes = [_vp, _vp, _vp, _vp]
         L.slgpu_get_particles.argtypes = [_vp, _i64] + [_vp] * 8
         L.slgpu_sample_groups.argtypes = [_vp, _i, _i, _i64, _vp, _vp, _vp]
+        L.slgpu_set_unstaged.argtypes = [_vp, _i]
         L.slgpu_opacity_vjp.argtypes = [_vp] + [_vp] * 8
         L.slgpu_timers.argtypes = [_vp, _vp, _i, _i]
         L.slgpu_stream.restype = _vp
@@ -376,6 +378,10 @@ class Context:
         self._check(self.L.slgpu_sample_groups(self.h, int(table), int(direct), len(r), _p(r),
                                                _p(cell), _p(out)))
         return out
+
+    def set_unstaged(self, on=True):
+        """verification hook: take the opacity build's and backward's unstaged kernels"""
+        self._check(self.L.slgpu_set_unstaged(self.h, int(bool(on))))
 
     def timers(self, reset=False):
         ms = np.zeros(12)

@@ -419,6 +419,40 @@ def test_opacity_vjp_precision_form(S, tables):
         assert err < 1e-5, k
 
 
+def test_opacity_unstaged_kernels_c1(S, tables):
+    """The opacity build and its backward through the kernels they take when the ion
+    populations do not fit in LDS (forced with slgpu_set_unstaged; no workload here has
+    enough ions to select them by size), against the staged kernels on the same inputs and
+    output gradients, in both precisions: FP64 within 1e-12, the precision form within 1e-5
+    of the largest value of each kind (its staged line sums run in ion order, the unstaged
+    ones in wavelength order)."""
+    say("case: opacity build and backward, unstaged against staged kernels, c1")
+    rng = np.random.default_rng(11)
+    for prec, tol in ((S.SLGPU_FP64, 1e-12), (S.SLGPU_MIXED, 1e-5)):
+        out = {}
+        for unstaged in (False, True):
+            ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1024, precision=prec)
+            if unstaged:
+                ctx.set_unstaged(True)
+            o = gpu_opacity(ctx, st)
+            if not unstaged:
+                Wc = rng.standard_normal(o["cap64"].shape) / np.abs(o["cap64"]).mean()
+                Wp = rng.standard_normal(w.nx) / np.abs(o["capgrey"]).mean()
+                Wr = rng.standard_normal(w.nx) / np.abs(o["capross"]).mean()
+            out[unstaged] = (o, ctx.opacity_vjp(Wc, Wp, Wr))
+            ctx.close()
+        (o0, g0), (o1, g1) = out[False], out[True]
+        for k in ("cap64", "capgrey", "capross"):
+            err = np.abs(o1[k] - o0[k]).max() / np.abs(o0[k]).max()
+            say(f"precision {prec} {k}: max |unstaged - staged| / max = {err:.2e}")
+            assert err < tol, k
+        for k in ("temp", "natom", "nelec", "natomfr", "oc"):
+            scale = np.abs(g0[k]).max()
+            err = np.abs(g1[k] - g0[k]).max() / scale
+            say(f"precision {prec} grad {k}: max |unstaged - staged| / max = {err:.2e}")
+            assert err < tol, k
+
+
 def test_sources_c1_per_particle(S, tables):
     say("case: sources c1")
     ctx, w, st, wl, lines = make_ctx(S, "c1", tables, npartmax=1 << 14, seed=5)
